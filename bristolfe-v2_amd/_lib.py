@@ -1,0 +1,212 @@
+"""ctypes binding of the C ABI in ``include/bfe.h`` (``libbfe_hip.so``).
+
+The library is the only compute path: every loader failure raises, there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libbfe_hip.so"
+
+BFE_OK, BFE_ERR_ARG, BFE_ERR_HIP, BFE_ERR_NOMEM, BFE_ERR_NUMERIC, BFE_ERR_INTERNAL = range(6)
+MODE_VEL_AT_CURRENT = 0
+MODE_VEL_AT_LAST_HALF = 1
+PREC_DOUBLE = 0
+PREC_SINGLE = 1
+
+# Every symbol the header declares.
+EXPORTS = (
+    "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_kernel_timing", "bfe_info_get",
+    "bfe_last_run_stats", "bfe_destroy", "bfe_last_error",
+)
+
+
+class BfeInfo(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64), ("nnz_k", C.c_int64), ("nnz_sell", C.c_int64),
+        ("n_interface", C.c_int64), ("nnz_interface", C.c_int64), ("nnz_xinv", C.c_int64),
+        ("damped", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
+        ("device", C.c_int32), ("device_bytes", C.c_int64),
+    ]
+
+
+class BfeRunStats(C.Structure):
+    _fields_ = [
+        ("loop_ms", C.c_double), ("steps", C.c_int64), ("launches", C.c_int64),
+        ("columns", C.c_int64), ("nonfinite", C.c_int64), ("step_kernel_ms", C.c_double),
+        ("step_kernel_samples", C.c_int64),
+    ]
+
+
+class BfeError(RuntimeError):
+    """Raised for any non-zero bfe_status."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"bfe status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+_p = C.POINTER
+_i64p = _p(C.c_int64)
+_i32p = _p(C.c_int32)
+_f64p = _p(C.c_double)
+
+
+def load() -> C.CDLL:
+    """Loads ``libbfe_hip.so``; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python bristolfe-v2_amd/build.py or __graft_entry__.build())")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.bfe_create.argtypes = [C.c_int64, _i64p, _i32p, _f64p, _f64p, _i64p, _i32p, _f64p,
+                               C.c_double, C.c_int32, C.c_int32, C.c_int32, _p(C.c_void_p)]
+    lib.bfe_create.restype = C.c_int
+    lib.bfe_run.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32, _i32p,
+                            C.c_int32, _f64p, C.c_int32, C.c_int32, _f64p, _i32p]
+    lib.bfe_run.restype = C.c_int
+    lib.bfe_run_batched.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, C.c_int32,
+                                    _f64p, C.c_int32, _i32p, C.c_int32, _f64p, _i32p]
+    lib.bfe_run_batched.restype = C.c_int
+    lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
+    lib.bfe_set_kernel_timing.restype = C.c_int
+    lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
+    lib.bfe_info_get.restype = C.c_int
+    lib.bfe_last_run_stats.argtypes = [C.c_void_p, _p(BfeRunStats)]
+    lib.bfe_last_run_stats.restype = C.c_int
+    lib.bfe_destroy.argtypes = [C.c_void_p]
+    lib.bfe_destroy.restype = C.c_int
+    lib.bfe_last_error.argtypes = []
+    lib.bfe_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != BFE_OK:
+        msg = load().bfe_last_error()
+        raise BfeError(status, msg.decode() if msg else "")
+
+
+def ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_p(ctype))
+
+
+class Plan:
+    """Owns one ``bfe_handle``: operator, coefficients and a HIP stream."""
+
+    def __init__(self, n, k_ptr, k_col, k_val, m, c_ptr, c_col, c_val, dt, mode, precision,
+                 device=0):
+        lib = load()
+        self._lib = lib
+        self.n = int(n)
+        arrs = dict(
+            k_ptr=np.ascontiguousarray(k_ptr, np.int64), k_col=np.ascontiguousarray(k_col, np.int32),
+            k_val=np.ascontiguousarray(k_val, np.float64), m=np.ascontiguousarray(m, np.float64),
+        )
+        if c_ptr is not None:
+            arrs.update(c_ptr=np.ascontiguousarray(c_ptr, np.int64),
+                        c_col=np.ascontiguousarray(c_col, np.int32),
+                        c_val=np.ascontiguousarray(c_val, np.float64))
+        else:
+            arrs.update(c_ptr=None, c_col=None, c_val=None)
+        if arrs["m"].shape[0] != self.n or arrs["k_ptr"].shape[0] != self.n + 1:
+            raise ValueError("inconsistent sizes for K / m")
+        h = C.c_void_p()
+        check(lib.bfe_create(
+            self.n, ptr(arrs["k_ptr"], C.c_int64), ptr(arrs["k_col"], C.c_int32),
+            ptr(arrs["k_val"], C.c_double), ptr(arrs["m"], C.c_double),
+            ptr(arrs["c_ptr"], C.c_int64), ptr(arrs["c_col"], C.c_int32),
+            ptr(arrs["c_val"], C.c_double), float(dt), int(mode), int(precision), int(device),
+            C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_scipy(cls, K, m, C_mat, dt, mode, precision, device=0):
+        K = K.tocsr()
+        c_args = (None, None, None)
+        if C_mat is not None and C_mat.nnz > 0:
+            Cc = C_mat.tocsr()
+            c_args = (Cc.indptr, Cc.indices, Cc.data)
+        return cls(K.shape[0], K.indptr, K.indices, K.data, m, *c_args, dt, mode, precision, device)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            check(self._lib.bfe_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        inf = BfeInfo()
+        check(self._lib.bfe_info_get(self._h, C.byref(inf)))
+        return {f: getattr(inf, f) for f, _ in BfeInfo._fields_}
+
+    def set_kernel_timing(self, every: int) -> None:
+        check(self._lib.bfe_set_kernel_timing(self._h, int(every)))
+
+    def stats(self) -> dict:
+        st = BfeRunStats()
+        check(self._lib.bfe_last_run_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in BfeRunStats._fields_}
+
+    def run(self, n_t, frc_idx, frcs, hist_idx, field_every=0, field_is_vel=1):
+        """0-based indices; frcs is (rows x n_t). Returns (hist, field, ti_start)."""
+        frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
+        hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
+        p, q = frc_idx.size, hist_idx.size
+        frcs = np.asarray(frcs, np.float64)
+        if frcs.ndim == 1:
+            frcs = frcs[None, :]
+        rows = frcs.shape[0] if p > 0 else 0
+        Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
+        hist = np.zeros((q, n_t), order="F")
+        n_frames = (n_t - 1) // field_every + 1 if field_every and field_every > 0 else 0
+        fld = np.zeros((self.n, n_frames), order="F") if n_frames else None
+        ti = C.c_int32()
+        check(self._lib.bfe_run(
+            self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
+            ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p),
+            int(field_every if n_frames else 0), int(field_is_vel),
+            fld.ctypes.data_as(_f64p) if fld is not None else None, C.byref(ti)))
+        return hist, fld, int(ti.value)
+
+    def run_batched(self, n_t, n_src, frc_idx, frc_src, frcs, hist_idx):
+        """Returns hist of shape (q, n_src, n_t) and ti_start."""
+        frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
+        frc_src = np.ascontiguousarray(frc_src, np.int32).ravel()
+        hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
+        p, q = frc_idx.size, hist_idx.size
+        frcs = np.asarray(frcs, np.float64)
+        if frcs.ndim == 1:
+            frcs = frcs[None, :]
+        rows = frcs.shape[0] if p > 0 else 0
+        Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
+        hist = np.zeros((q, n_src, n_t), order="F")
+        ti = C.c_int32()
+        check(self._lib.bfe_run_batched(
+            self._h, int(n_t), int(n_src), ptr(frc_idx, C.c_int32), ptr(frc_src, C.c_int32), p,
+            Fcol.ctypes.data_as(_f64p), rows, ptr(hist_idx, C.c_int32), q,
+            hist.ctypes.data_as(_f64p), C.byref(ti)))
+        return hist, int(ti.value)

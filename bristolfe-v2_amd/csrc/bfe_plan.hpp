@@ -1,0 +1,76 @@
+// Host-side FP64 set-up of the explicit march (no device code here).
+//
+// Follows code/internal/fn_explicit_dynamic_solver_v6.m:122-139: the lumped
+// mass m = sum(M), and the update u = A f + B u1 + D u2 with
+//   'vel at curent time step':  X = I + dt/2 M^-1 C,
+//        A = X \ dt^2 M^-1,  B = X \ (2I - dt^2 M^-1 K),  D = X \ (-I + dt/2 M^-1 C)
+//   'vel at last half time step':
+//        A = dt^2 M^-1,  B = 2I - dt M^-1 C - dt^2 M^-1 K,  D = dt M^-1 C - I
+// Instead of forming A, B, D, every row keeps
+//   u_i = beta_i u1_i + gamma_i u2_i - sum_j (alpha_i K_ij) u1_j + alpha_i f_i
+// which is exact for rows where C has no off-diagonal entries. Rows coupled by
+// off-diagonal C (ASI2D2 fluid-solid interface, fn_el_ASI2D2.m:60-76) form the
+// set I; they get an explicit correction and, in the default mode, the exact
+// solve u_I = X_II^-1 y_I with X_II^-1 computed here in FP64.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bfe {
+
+struct Csr {
+  int64_t n = 0;
+  std::vector<int64_t> ptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  int64_t nnz() const { return ptr.empty() ? 0 : ptr.back(); }
+};
+
+// Sliced ELLPACK with slice height 64 (one wavefront), no row sorting.
+// Entry k of row r lives at off[r/64] + 64*k + r%64. Padding entries point
+// at the row itself with value 0.
+struct Sell64 {
+  int64_t n = 0;
+  int64_t n_slices = 0;
+  std::vector<int64_t> off;  // n_slices + 1, in entries
+  std::vector<int32_t> col;
+  std::vector<double> val;
+};
+
+struct Plan {
+  int64_t n = 0;
+  int mode = 0;
+  double dt = 0.0;
+  std::vector<double> alpha, beta, gamma;
+  bool damped = false;
+  int64_t nnz_k = 0;
+  Sell64 op;  // alpha-scaled K
+  // Interface rows (sorted) and their off-diagonal C scaled by 1/m_i.
+  std::vector<int32_t> iface;         // global row of interface entry k
+  std::vector<int32_t> iface_pos;     // n entries: k or -1
+  Csr coff;                           // rows k over global columns
+  double sq = 0.0, sr = 0.0;          // y += sq*Coff*u1 + sr*Coff*u2
+  Csr xinv;                           // rows/cols over interface positions
+  bool has_xinv = false;
+};
+
+// Normalises a CSR (sorted columns, duplicates summed, exact zeros removed)
+// and validates it. Throws std::invalid_argument on malformed input.
+Csr normalise_csr(int64_t n, const int64_t* ptr, const int32_t* col,
+                  const double* val, const char* name);
+
+// Builds the whole plan. Throws std::invalid_argument (bad input) or
+// std::domain_error (numerical problem, e.g. m_i == 0).
+Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C,
+                double dt, int mode);
+
+// Exact inverse of the sparse, diagonally dominated interface block, column
+// by column on growing graph neighbourhoods (dense LU with partial pivoting
+// on each neighbourhood) until the neighbourhood boundary carries less than
+// 1e-20 of the column; entries below drop_tol of a column's largest entry
+// are dropped. Exposed for tests.
+Csr truncated_inverse(const Csr& X, double drop_tol);
+
+}  // namespace bfe

@@ -1,0 +1,171 @@
+/*
+ * bfe.h — C ABI of the MI355X (gfx950) explicit time-march of BristolFE-v2.
+ *
+ * The library replaces the time-marching loop of
+ *   code/internal/fn_explicit_dynamic_solver_v6.m:1-221
+ * which code/fn_BristolFE_v2.m:120-122 calls with the assembled K, C, M.
+ * A MEX shim (or any FFI) passes exactly those inputs: compressed sparse rows
+ * of K and C (both symmetric, so MATLAB's CSC arrays Jc/Ir/Pr can be passed
+ * unchanged as row pointers/columns/values), the lumped mass m = sum(M)
+ * (v6:122-123), and dt = time(2) - time(1) (v6:93).
+ *
+ * Conventions
+ *   - Every index crossing this ABI is 0-based. Row pointers are int64,
+ *     column and DOF indices int32.
+ *   - Dense time data are column-major, as MATLAB stores them:
+ *       frcs  : frc_rows x n_t,  element (r, t) at frcs[r + frc_rows * t]
+ *       hist  : q x n_t
+ *       field : n x n_frames
+ *   - The caller allocates every output buffer; the handle owns device
+ *     memory and one HIP stream. Inputs are borrowed for the call only.
+ *   - Every function returns a bfe_status; nothing throws across the ABI.
+ *     bfe_last_error() returns the message of the calling thread's last
+ *     failure.
+ *   - Handles are independent: calls on different handles may run from
+ *     different threads; calls on one handle must not overlap.
+ */
+#ifndef BFE_H
+#define BFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum bfe_status {
+  BFE_OK = 0,
+  BFE_ERR_ARG = 1,     /* malformed input (shape, index range, non-square) */
+  BFE_ERR_HIP = 2,     /* HIP runtime failure (no device, launch error)    */
+  BFE_ERR_NOMEM = 3,   /* device or host allocation failed                 */
+  BFE_ERR_NUMERIC = 4, /* zero lumped mass, singular interface block,      */
+                       /* non-finite values in the march                   */
+  BFE_ERR_INTERNAL = 5
+} bfe_status;
+
+/* solver_mode strings of code/fn_BristolFE_v2.m:28 and
+ * fn_explicit_dynamic_solver_v6.m:130-139 */
+#define BFE_MODE_VEL_AT_CURRENT 0   /* 'vel at curent time step' (default) */
+#define BFE_MODE_VEL_AT_LAST_HALF 1 /* 'vel at last half time step'        */
+
+/* solver_precision of fn_FE_entry_point.m:31 and v6:142-158 */
+#define BFE_PREC_DOUBLE 0
+#define BFE_PREC_SINGLE 1
+
+typedef struct bfe_plan_s* bfe_handle;
+
+/* Static description of a plan, for logging and roofline accounting. */
+typedef struct bfe_info {
+  int64_t n;             /* DOF                                          */
+  int64_t nnz_k;         /* stored entries of K (after dropping zeros)   */
+  int64_t nnz_sell;      /* entries of the SELL-64 operator incl. padding */
+  int64_t n_interface;   /* rows coupled through off-diagonal C          */
+  int64_t nnz_interface; /* entries of the interface coupling C_off      */
+  int64_t nnz_xinv;      /* entries of the truncated X_II^-1             */
+  int32_t damped;        /* 1 if any row carries a damping term          */
+  int32_t mode;
+  int32_t precision;
+  int32_t device;
+  int64_t device_bytes;  /* operator + coefficient bytes on the device   */
+} bfe_info;
+
+/* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
+ * on the handle's stream around the time loop only (the interval the
+ * reference brackets at v6:162-220). */
+typedef struct bfe_run_stats {
+  double loop_ms;        /* HIP-event time of the whole time loop        */
+  int64_t steps;         /* time steps executed (n_t - ti_start + 1)     */
+  int64_t launches;      /* kernel launches inside the loop              */
+  int64_t columns;       /* right-hand sides marched together            */
+  int64_t nonfinite;     /* non-finite values found in the history output */
+                         /* (an unstable dt diverges silently in v6)      */
+  double step_kernel_ms; /* summed HIP-event time of the sampled step-    */
+                         /* kernel launches (see bfe_set_kernel_timing)   */
+  int64_t step_kernel_samples;
+} bfe_run_stats;
+
+/*
+ * bfe_create — set up a march.
+ * Replaces v6:122-159 (lumped mass, A/B/D formation, device upload).
+ *   n          number of global DOF (size of K, C, M)
+ *   k_*        CSR of K (n x n); duplicate (row, col) entries are summed
+ *   m          lumped mass per DOF, sum(M) (v6:122); every entry non-zero
+ *   c_*        CSR of C (n x n); may be NULL/empty (C = 0). Diagonal entries
+ *              are absorbing-layer damping; off-diagonal entries are the
+ *              fluid-solid coupling of ASI2D2 elements
+ *              (code/internal/elements/fn_el_ASI2D2.m:60-76)
+ *   dt         time step
+ *   mode       BFE_MODE_*
+ *   precision  BFE_PREC_*; single keeps FP64 setup and marches in FP32
+ *   device     HIP device ordinal
+ * For mode 'vel at curent time step' the implicit part
+ * X = I + dt/2 M^-1 C (v6:136-138) is applied exactly: diagonal rows are
+ * folded into per-row coefficients, the interface block X_II is inverted on
+ * the host in FP64 and applied per step (entries below 1e-18 of a column's
+ * largest are dropped).
+ */
+bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
+                      const double* k_val, const double* m,
+                      const int64_t* c_rowptr, const int32_t* c_col,
+                      const double* c_val, double dt, int32_t mode,
+                      int32_t precision, int32_t device, bfe_handle* out);
+
+/*
+ * bfe_run — march n_t time samples from rest.
+ * Replaces the loop v6:162-218 and its outputs:
+ *   frc_idx[p], frcs (frc_rows x n_t)  forcing DOFs and signals; frc_rows is
+ *              1 (one signal broadcast to every DOF, v6:182 with a scalar
+ *              right-hand side) or p. Duplicate DOFs: the last wins, as in
+ *              MATLAB's f(idx) = v.
+ *   hist_idx[q], hist_out (q x n_t)    monitored DOFs, u after each update
+ *              (v6:195-197); columns before ti_start stay 0.
+ *   field_every  frames at ti = 1, 1+N, ... (v6:112-114); <= 0 means none.
+ *   field_is_vel 1: frame = (u - u_prev)/dt, 0: frame = u (v6:64-69, 200-208)
+ *   field_out (n x n_frames) may be NULL when field_every <= 0.
+ *   ti_start_out  1-based first step with non-zero forcing (v6:163-166);
+ *              n_t + 1 when the forcing is all zero (nothing is marched).
+ */
+bfe_status bfe_run(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                   const double* frcs, int32_t frc_rows, const int32_t* hist_idx,
+                   int32_t q, double* hist_out, int32_t field_every,
+                   int32_t field_is_vel, double* field_out,
+                   int32_t* ti_start_out);
+
+/*
+ * bfe_run_batched — march n_src independent load cases on the same operator
+ * in one launch sequence (sparse x multivector). Replaces the sequential
+ * per-step loop of code/fn_BristolFE_v2.m:72-123 as driven by
+ * subdoms/fn_run_main_model.m:101-115 (one step per transmitter).
+ *   frc_idx[p], frc_src[p]  forcing DOF and the source column it belongs to
+ *   frcs (frc_rows x n_t)   signals; frc_rows = 1 broadcasts one signal to
+ *                           every forcing entry, else frc_rows = p
+ *   hist_idx[q]             monitored DOFs, recorded for every source
+ *   hist_out (q x n_src x n_t)  element (k, s, t) at k + q*(s + n_src*t)
+ * All sources start at the same ti_start (the first step at which any
+ * forcing is non-zero). No field output.
+ */
+bfe_status bfe_run_batched(bfe_handle h, int32_t n_t, int32_t n_src,
+                           const int32_t* frc_idx, const int32_t* frc_src,
+                           int32_t p, const double* frcs, int32_t frc_rows,
+                           const int32_t* hist_idx, int32_t q, double* hist_out,
+                           int32_t* ti_start_out);
+
+/*
+ * bfe_set_kernel_timing — sample the duration of the main step kernel.
+ * No counterpart in the reference (its only timing is the wall clock of
+ * v6:220). every > 0 brackets every every-th step-kernel launch with HIP
+ * events on the handle's stream; the next runs report the sum and count in
+ * bfe_run_stats. every <= 0 switches sampling off (the default).
+ */
+bfe_status bfe_set_kernel_timing(bfe_handle h, int32_t every);
+
+bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
+bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);
+bfe_status bfe_destroy(bfe_handle h);
+const char* bfe_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BFE_H */

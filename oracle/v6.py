@@ -1,0 +1,125 @@
+"""Literal restatement of code/internal/fn_explicit_dynamic_solver_v6.m.
+
+ORACLE — test infrastructure only (see oracle/__init__.py).
+
+Line map:
+  v6:93        dt = time(2) - time(1)
+  v6:112-120   field frames at ti = 1:N:n_t, zero-initialised
+  v6:122-123   diag_M = spdiags(sum(M)), inv_M = spdiags(1 ./ sum(M))
+  v6:130-139   A, B, D per solver mode; 'X \\ Y' for the current-step mode
+  v6:163-166   ti_start = first column with any non-zero forcing
+  v6:180-213   f(idx) = F(:, ti); u = A*f + B*u1 + D*u2; hist(:, ti) = u(h);
+               frame (u - u1)/dt or u; shift
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.linalg as sla
+import scipy.sparse as sp
+
+FIELD_U_TYPES = {"mean(u1)", "mean(u2)", "mean(u3)", "curl(u)", "div(u)", "raw(u)"}
+
+
+def left_divide(X, Y):
+    """X \\ Y for sparse X that is diagonal outside a (small) coupled block.
+
+    Block Gaussian elimination: rows/columns with off-diagonal entries of X
+    form the block I, solved densely with LU; every other row is divided by
+    its diagonal. The same linear algebra as MATLAB's sparse '\\'.
+    """
+    X = sp.csr_matrix(X)
+    Y = sp.csr_matrix(Y)
+    n = X.shape[0]
+    coo = X.tocoo()
+    nz_off = (coo.row != coo.col) & (coo.data != 0)
+    coupled = np.zeros(n, dtype=bool)
+    coupled[coo.row[nz_off]] = True
+    coupled[coo.col[nz_off]] = True
+    d = X.diagonal()
+    scale = np.zeros(n)
+    scale[~coupled] = 1.0 / d[~coupled]
+    out = (sp.diags(scale) @ Y).tocoo()
+    rows, cols, vals = [out.row], [out.col], [out.data]
+    I = np.nonzero(coupled)[0]
+    if I.size:
+        XII = X[I][:, I].toarray()
+        YI = Y[I].toarray()
+        Z = sla.lu_solve(sla.lu_factor(XII), YI)
+        zr, zc = np.nonzero(Z)
+        rows.append(I[zr])
+        cols.append(zc)
+        vals.append(Z[zr, zc])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                         shape=(n, Y.shape[1]))
+
+
+def abd_matrices(K, C, M, dt, solver_mode):
+    ndf = K.shape[0]
+    m = np.asarray(M.sum(axis=0)).ravel()
+    inv_M = sp.diags(1.0 / m)
+    I = sp.eye(ndf, format="csr")
+    if solver_mode == "vel at last half time step":
+        A = dt ** 2 * inv_M
+        B = 2 * I - dt * (inv_M @ C) - dt ** 2 * (inv_M @ K)
+        D = dt * (inv_M @ C) - I
+    elif solver_mode in ("vel at curent time step", "vel at current time step"):
+        X = I + dt / 2 * (inv_M @ C)
+        A = left_divide(X, dt ** 2 * inv_M)
+        B = left_divide(X, 2 * I - dt ** 2 * (inv_M @ K))
+        D = left_divide(X, -I + dt / 2 * (inv_M @ C))
+    else:
+        raise ValueError(solver_mode)
+    return sp.csr_matrix(A), sp.csr_matrix(B), sp.csr_matrix(D)
+
+
+def ti_start_of(forcing_functions):
+    F = np.atleast_2d(np.asarray(forcing_functions, dtype=np.float64))
+    nz = np.nonzero(np.abs(F).sum(axis=0) != 0)[0]
+    return int(nz[0]) + 1 if nz.size else None
+
+
+def explicit_dynamic_solver_v6(K, C, M, time, forcing_indices, forcing_functions,
+                               history_indices, field_output_every_n_frames=np.inf,
+                               field_output_type="KE", solver_mode="vel at curent time step",
+                               abd=None):
+    """1-based indices as in MATLAB. Returns (hist, fld, fld_time, ti_start)."""
+    time = np.asarray(time, dtype=np.float64).ravel()
+    n_t = time.size
+    ndf = K.shape[0]
+    dt = time[1] - time[0]
+    fi = np.asarray(forcing_indices, dtype=np.int64).ravel() - 1
+    F = np.atleast_2d(np.asarray(forcing_functions, dtype=np.float64))
+    hi = np.asarray(history_indices, dtype=np.int64).ravel() - 1
+    hist = np.zeros((hi.size, n_t))
+    if np.isinf(field_output_every_n_frames):
+        frames = np.zeros(0, dtype=np.int64)
+    else:
+        frames = np.arange(1, n_t + 1, int(field_output_every_n_frames))
+    fld = np.zeros((ndf, frames.size))
+    fld_time = np.zeros(frames.size)
+    is_vel = field_output_type not in FIELD_U_TYPES
+    A, B, D = abd if abd is not None else abd_matrices(K, C, M, dt, solver_mode)
+    u1 = np.zeros(ndf)
+    u2 = np.zeros(ndf)
+    f = np.zeros(ndf)
+    ti_start = ti_start_of(F) if fi.size else None
+    if ti_start is None:
+        return hist, fld, fld_time, n_t + 1
+    frame_of = {int(t): k for k, t in enumerate(frames)}
+    for ti in range(ti_start, n_t + 1):
+        col = F[:, ti - 1]
+        if col.size == 1:
+            f[fi] = col[0]
+        else:
+            for k in range(fi.size):  # sequential assignment: last duplicate wins
+                f[fi[k]] = col[k]
+        u = A @ f + B @ u1 + D @ u2
+        if hi.size:
+            hist[:, ti - 1] = u[hi]
+        if ti in frame_of:
+            k = frame_of[ti]
+            fld_time[k] = time[ti - 1]
+            fld[:, k] = (u - u1) / dt if is_vel else u
+        u2 = u1
+        u1 = u
+    return hist, fld, fld_time, ti_start
