@@ -38,19 +38,33 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_workload(name):
+def build_workload(name, n_t=None):
     import __graft_entry__ as ge
     ge.load_package()
     from bristolfe_v2_amd import configs
+    kw = {} if n_t is None else {"n_t": n_t}
     if name == "coupled_2d":
-        return configs.coupled_2d()
+        return configs.coupled_2d(**kw)
     if name == "plate_2d":
-        return configs.plate_2d()
+        return configs.plate_2d(**kw)
     m = re.fullmatch(r"coupled_2d@([0-9.]+)", name)
     if m:  # reduced element size, for quick runs
         h = float(m.group(1)) * 1e-3
-        return configs.coupled_2d(el_size=h)
+        return configs.coupled_2d(el_size=h, **kw)
     raise SystemExit(f"unknown workload {name}")
+
+
+def copy_probe(gib=2.0):
+    """A plain device copy of known size, for calibrating rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (run under rocprofv3 --pmc)."""
+    import torch
+    n = int(gib * (1 << 30)) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 8, "dispatches": 3}))
 
 
 def algorithmic_bytes_per_dof(nnz, n, damped):
@@ -68,7 +82,7 @@ def read_traffic(workload):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(c, budget_s=15.0):
+def cpu_baseline(c, hi, budget_s=15.0):
     from oracle.march import OracleMarch
     threads = min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
@@ -78,7 +92,6 @@ def cpu_baseline(c, budget_s=15.0):
     om = OracleMarch(K, C, M, c["dt"])
     setup = time.perf_counter() - t0
     fi = np.asarray(c["frc_gi"]) - 1
-    hi = np.asarray(c["hist_gi"]) - 1
     # probe a few steps, then run a sample sized to the budget
     t0 = time.perf_counter()
     om.run(n_t, fi, c["frcs"], hi, t_end=5)
@@ -102,14 +115,21 @@ def main():
     ap.add_argument("--workload", default="coupled_2d")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timing-every", type=int, default=16)
+    ap.add_argument("--time-steps", type=int, default=None,
+                    help="override the workload's number of time steps (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 2 GiB device copy (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the march is single-GPU; --gpus {args.gpus} "
               "is not supported)", file=sys.stderr)
         sys.exit(2)
 
     t_setup = time.perf_counter()
-    c = build_workload(args.workload)
+    c = build_workload(args.workload, args.time_steps)
     log(f"workload {c['name']}: n={c['K'].shape[0]} nnz={c['K'].nnz} n_t={len(c['time'])} "
         f"setup {time.perf_counter() - t_setup:.1f} s")
     from bristolfe_v2_amd.solver import make_plan
@@ -180,11 +200,13 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        cb, hcpu, sample_steps = cpu_baseline(c)
+        # the sample's check monitors: the forcing DOFs plus the workload's monitors
+        hchk = np.concatenate([fi, hi])
+        cb, hcpu, sample_steps = cpu_baseline(c, hchk)
         out["cpu_baseline"] = cb
-        a = hist[:, :sample_steps]
+        hg, _, _ = plan.run(sample_steps, fi, F[:, :sample_steps], hchk)
         b = hcpu[:, :sample_steps]
-        out["cpu_gpu_rel_l2_on_sample"] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        out["cpu_gpu_rel_l2_on_sample"] = float(np.linalg.norm(hg - b) / max(np.linalg.norm(b), 1e-300))
     else:
         out["cpu_baseline"] = None
     plan.close()

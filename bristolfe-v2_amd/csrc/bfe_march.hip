@@ -4,10 +4,12 @@
 //   f(idx) = F(:, ti);  u = A f + B u1 + D u2;  hist(:, ti) = u(h_idx);
 //   [fld(:, fi) = (u - u1)/dt];  u2 = u1;  u1 = u
 // becomes one fused, HBM-streaming kernel over all rows (k_step) plus, only
-// when the model has fluid-solid interface rows, one single-workgroup kernel
-// (k_iface) that applies the interface coupling and X_II^-1. Monitor gathers
-// and field frames of step t-1 are written by the step-t kernel from the
-// u1/u2 values it already reads, so they cost no extra launch or pass.
+// when the model has fluid-solid interface rows in the current-step mode, one
+// small kernel (k_iface) that applies the implicit interface block
+// u_I = X_II^-1 y_I to the right-hand side y_I that k_step left in a compact
+// buffer.
+// Monitor gathers and field frames of step t-1 are written by the step-t
+// kernel from the u1/u2 values it already reads, so they cost no extra pass.
 //
 // Operator layout (host set-up in bfe_plan.cpp): SELL-64 of alpha-scaled K.
 // A wavefront owns one 64-row slice; lane l streams row 64*s+l, so the value
@@ -17,8 +19,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdarg>
-#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -31,8 +31,9 @@
 
 namespace {
 
-constexpr int kBlock = 256;    // rows per workgroup in k_step (4 slices)
-constexpr int kIfaceThreads = 1024;
+constexpr int kBlock = 256;              // threads per workgroup
+constexpr int kRowsSingle = kBlock;      // rows per workgroup, single source
+constexpr int kRowsMulti = kBlock / 64;  // rows per workgroup, multi source
 
 struct hip_failure : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -57,24 +58,52 @@ __device__ __forceinline__ int64_t xcd_block(int64_t bid, int64_t nb) {
   return x * q + (x < r ? x : r) + j;
 }
 
+// Per-workgroup lists (forcing, monitors, interface rows) let a thread find
+// the few special rows of its block without a per-row array in HBM.
+struct BlockLists {
+  const int32_t* fptr;    // forcing: block b owns [fptr[b], fptr[b+1])
+  const int32_t* f_row;
+  const int32_t* f_src;   // source column (multi-source)
+  const int32_t* f_frow;  // row of the signal matrix F
+  const int32_t* mptr;    // monitors
+  const int32_t* m_row;
+  const int32_t* m_k;
+  const int32_t* iptr;    // interface rows
+  const int32_t* i_row;
+  const int32_t* i_k;
+};
+
+// Interface rows k (global row iface[k]): off-diagonal C scaled by 1/m,
+// the gamma correction and the solver-mode factors. k_step turns w into
+//   y_k = w_k + gfix_k u2_r + sq sum_j Coff_kj u1_j + sr sum_j Coff_kj u2_j
+// which is final in the last-half-step mode and the right-hand side of
+// X_II u_I = y_I (applied by k_iface) in the current-step mode.
+template <typename T>
+struct IfaceRows {
+  const int64_t* cptr;
+  const int32_t* ccol;
+  const T* cval;
+  const T* gfix;
+  T sq, sr;
+  T* wI;  // nullptr: no X_II solve, y is final
+};
+
 // ---------------------------------------------------------------------------
 // Single right-hand side step.
-//   u[r] = beta[r] u1[r] + gamma[r] u2[r] - sum_k val[k] u1[col[k]]  (+ forcing)
-// DAMPED = false: beta = 2, gamma = -1 for every row (no arrays read).
+//   u[r] = beta[r] u1[r] + (1 - beta[r]) u2[r] - sum_k val[k] u1[col[k]]  (+ forcing)
+// (gamma = 1 - beta holds on every row outside the interface; interface rows
+// add their correction from IfaceRows). DAMPED = false: beta = 2 everywhere
+// and no beta array is read.
 template <typename T, bool DAMPED>
 __global__ __launch_bounds__(kBlock) void k_step(
     int64_t n, int64_t nblk, const int64_t* __restrict__ off,
-    const int32_t* __restrict__ col, const T* __restrict__ val,
-    const T* __restrict__ beta, const T* __restrict__ gamma,
+    const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
-    const int32_t* __restrict__ fptr, const int32_t* __restrict__ f_row,
-    const T* __restrict__ f_alpha, const int32_t* __restrict__ f_frow,
-    const T* __restrict__ F, int32_t F_rows, int32_t t,
-    const int32_t* __restrict__ mptr, const int32_t* __restrict__ m_row,
-    const int32_t* __restrict__ m_k, T* __restrict__ hist_prev,
-    T* __restrict__ fld_prev, int32_t fld_is_vel, T dt) {
+    BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
+    int32_t t, T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
+    IfaceRows<T> I) {
   const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t row = lb * kBlock + threadIdx.x;
+  const int64_t row = lb * kRowsSingle + threadIdx.x;
   if (row >= n) return;
   const int64_t sl = row >> 6;
   const int lane = static_cast<int>(row & 63);
@@ -88,15 +117,30 @@ __global__ __launch_bounds__(kBlock) void k_step(
   if (p < e) acc0 += val[p] * u1[col[p]];
   const T a = u1[row], b = u2[row];
   T un;
-  if constexpr (DAMPED) un = beta[row] * a + gamma[row] * b - (acc0 + acc1);
-  else un = (T(2) * a - b) - (acc0 + acc1);
-  for (int32_t i = fptr[lb], ie = fptr[lb + 1]; i < ie; ++i)
-    if (f_row[i] == row) un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + f_frow[i]];
+  if constexpr (DAMPED) {
+    const T be = beta[row];
+    un = be * a + (T(1) - be) * b - (acc0 + acc1);  // gamma = 1 - beta
+  } else {
+    un = (T(2) * a - b) - (acc0 + acc1);
+  }
+  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i)
+    if (L.f_row[i] == row) un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i)
+    if (L.i_row[i] == row) {
+      const int32_t k = L.i_k[i];
+      T c1 = T(0), c2 = T(0);
+      for (int64_t q = I.cptr[k], qe = I.cptr[k + 1]; q < qe; ++q) {
+        c1 += I.cval[q] * u1[I.ccol[q]];
+        c2 += I.cval[q] * u2[I.ccol[q]];
+      }
+      un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
+      if (I.wI != nullptr) I.wI[k] = un;
+    }
   u[row] = un;
   if (fld_prev != nullptr) fld_prev[row] = fld_is_vel ? (a - b) / dt : a;
   if (hist_prev != nullptr)
-    for (int32_t i = mptr[lb], ie = mptr[lb + 1]; i < ie; ++i)
-      if (m_row[i] == row) hist_prev[m_k[i]] = a;
+    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i)
+      if (L.m_row[i] == row) hist_prev[L.m_k[i]] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -106,16 +150,12 @@ __global__ __launch_bounds__(kBlock) void k_step(
 template <typename T, bool DAMPED>
 __global__ __launch_bounds__(kBlock) void k_step_multi(
     int64_t n, int32_t S, int64_t nblk, const int64_t* __restrict__ off,
-    const int32_t* __restrict__ col, const T* __restrict__ val,
-    const T* __restrict__ beta, const T* __restrict__ gamma,
+    const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
-    const int32_t* __restrict__ fptr, const int32_t* __restrict__ f_row,
-    const int32_t* __restrict__ f_src, const T* __restrict__ f_alpha,
-    const int32_t* __restrict__ f_frow, const T* __restrict__ F, int32_t F_rows,
-    int32_t t, const int32_t* __restrict__ mptr, const int32_t* __restrict__ m_row,
-    const int32_t* __restrict__ m_k, int32_t q, T* __restrict__ hist_prev) {
+    BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
+    int32_t t, int32_t q, T* __restrict__ hist_prev, IfaceRows<T> I) {
   const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t row = lb * (kBlock / 64) + (threadIdx.x >> 6);
+  const int64_t row = lb * kRowsMulti + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n || lane >= S) return;
   const int64_t sl = row >> 6;
@@ -126,62 +166,69 @@ __global__ __launch_bounds__(kBlock) void k_step_multi(
   const int64_t me = row * S + lane;
   const T a = u1[me], b = u2[me];
   T un;
-  if constexpr (DAMPED) un = beta[row] * a + gamma[row] * b - acc;
-  else un = (T(2) * a - b) - acc;
-  for (int32_t i = fptr[lb], ie = fptr[lb + 1]; i < ie; ++i)
-    if (f_row[i] == row && f_src[i] == lane)
-      un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + f_frow[i]];
+  if constexpr (DAMPED) {
+    const T be = beta[row];
+    un = be * a + (T(1) - be) * b - acc;
+  } else {
+    un = (T(2) * a - b) - acc;
+  }
+  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i)
+    if (L.f_row[i] == row && L.f_src[i] == lane)
+      un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i)
+    if (L.i_row[i] == row) {
+      const int32_t k = L.i_k[i];
+      T c1 = T(0), c2 = T(0);
+      for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
+        const int64_t c = static_cast<int64_t>(I.ccol[p]) * S + lane;
+        c1 += I.cval[p] * u1[c];
+        c2 += I.cval[p] * u2[c];
+      }
+      un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
+      if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un;
+    }
   u[me] = un;
   if (hist_prev != nullptr)
-    for (int32_t i = mptr[lb], ie = mptr[lb + 1]; i < ie; ++i)
-      if (m_row[i] == row) hist_prev[m_k[i] + static_cast<int64_t>(q) * lane] = a;
+    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i)
+      if (L.m_row[i] == row) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a;
 }
 
 // ---------------------------------------------------------------------------
-// Interface rows (one workgroup). For interface entry k (global row r):
-//   y = w_r + sq * sum_j Coff_kj u1_j + sr * sum_j Coff_kj u2_j (+ forcing)
-//   u_r = sum_l Xinv_kl y_l        (identity when xptr == nullptr)
-// w_r is what k_step wrote to u[r] with the interface-row coefficients.
+// Interface solve of the current-step mode: u[iface[k]] = sum_l Xinv_kl y_l
+// with y = wI from k_step (compact, L2-resident). Single source: 16 lanes
+// share one row and reduce with cross-lane shuffles, so each lane walks only
+// a few of the row's entries. Reads wI, writes interface rows of u only.
+constexpr int kIfaceLanes = 16;
+
 template <typename T>
-__global__ __launch_bounds__(kIfaceThreads) void k_iface(
-    int32_t nI, int32_t S, const int32_t* __restrict__ iface,
-    const int64_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
-    const T* __restrict__ cval, T sq, T sr, const int64_t* __restrict__ xptr,
-    const int32_t* __restrict__ xcol, const T* __restrict__ xval,
-    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
-    T* __restrict__ y, int32_t nfi, const int32_t* __restrict__ fi_k,
-    const int32_t* __restrict__ fi_src, const T* __restrict__ fi_alpha,
-    const int32_t* __restrict__ fi_frow, const T* __restrict__ F, int32_t F_rows,
-    int32_t t) {
-  const int64_t tot = static_cast<int64_t>(nI) * S;
-  for (int64_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int64_t k = idx / S, s = idx - k * S;
-    const int64_t r = iface[k];
-    T a1 = T(0), a2 = T(0);
-    for (int64_t p = cptr[k], pe = cptr[k + 1]; p < pe; ++p) {
-      const int64_t c = static_cast<int64_t>(ccol[p]) * S + s;
-      a1 += cval[p] * u1[c];
-      a2 += cval[p] * u2[c];
-    }
-    y[idx] = u[r * S + s] + sq * a1 + sr * a2;
-  }
-  __syncthreads();
-  for (int32_t i = threadIdx.x; i < nfi; i += blockDim.x)
-    y[static_cast<int64_t>(fi_k[i]) * S + fi_src[i]] +=
-        fi_alpha[i] * F[static_cast<int64_t>(t) * F_rows + fi_frow[i]];
-  __syncthreads();
-  for (int64_t idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int64_t k = idx / S, s = idx - k * S;
-    T acc;
-    if (xptr != nullptr) {
-      acc = T(0);
-      for (int64_t p = xptr[k], pe = xptr[k + 1]; p < pe; ++p)
-        acc += xval[p] * y[static_cast<int64_t>(xcol[p]) * S + s];
-    } else {
-      acc = y[idx];
-    }
-    u[static_cast<int64_t>(iface[k]) * S + s] = acc;
-  }
+__global__ __launch_bounds__(kBlock) void k_iface(
+    int32_t nI, const int32_t* __restrict__ iface, const int64_t* __restrict__ xptr,
+    const int32_t* __restrict__ xcol, const T* __restrict__ xval, const T* __restrict__ wI,
+    T* __restrict__ u) {
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t k = gid / kIfaceLanes;
+  const int j = static_cast<int>(gid % kIfaceLanes);
+  T acc = T(0);
+  if (k < nI)
+    for (int64_t p = xptr[k] + j, pe = xptr[k + 1]; p < pe; p += kIfaceLanes)
+      acc += xval[p] * wI[xcol[p]];
+  for (int o = kIfaceLanes / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, kIfaceLanes);
+  if (k < nI && j == 0) u[iface[k]] = acc;
+}
+
+// Multi-source: one thread per (k, s); wI and u rows are S-contiguous.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_iface_multi(
+    int32_t nI, int32_t S, const int32_t* __restrict__ iface, const int64_t* __restrict__ xptr,
+    const int32_t* __restrict__ xcol, const T* __restrict__ xval, const T* __restrict__ wI,
+    T* __restrict__ u) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<int64_t>(nI) * S) return;
+  const int64_t k = idx / S, s = idx - k * S;
+  T acc = T(0);
+  for (int64_t p = xptr[k], pe = xptr[k + 1]; p < pe; ++p)
+    acc += xval[p] * wI[static_cast<int64_t>(xcol[p]) * S + s];
+  u[static_cast<int64_t>(iface[k]) * S + s] = acc;
 }
 
 // Monitors (and, single-source, the field frame) of the last step.
@@ -232,16 +279,43 @@ struct DevBuf {
   template <typename U> U* as() const { return static_cast<U*>(p); }
 };
 
+// Blocking copy: the source is often a temporary, and the data must be on
+// the device before any kernel of the handle's stream reads it.
 template <typename U>
-DevBuf upload(const std::vector<U>& v, hipStream_t st) {
+DevBuf upload(const std::vector<U>& v) {
   DevBuf b(v.size() * sizeof(U));
-  if (!v.empty()) BFE_HIP(hipMemcpyAsync(b.p, v.data(), b.bytes, hipMemcpyHostToDevice, st));
+  if (!v.empty()) BFE_HIP(hipMemcpy(b.p, v.data(), b.bytes, hipMemcpyHostToDevice));
   return b;
 }
 
 template <typename T>
 std::vector<T> cast_vec(const std::vector<double>& v) {
   return std::vector<T>(v.begin(), v.end());
+}
+
+// Groups (row, payload) entries by block of `rows_per_block` rows.
+struct Grouped {
+  std::vector<int32_t> ptr, row, a, b;
+};
+
+Grouped group_by_block(int64_t nblk, int64_t rows_per_block, const std::vector<int32_t>& rows,
+                       const std::vector<int32_t>* a, const std::vector<int32_t>* b) {
+  Grouped G;
+  G.ptr.assign(nblk + 1, 0);
+  for (int32_t r : rows) G.ptr[r / rows_per_block + 1]++;
+  for (int64_t k = 0; k < nblk; ++k) G.ptr[k + 1] += G.ptr[k];
+  const size_t m = rows.size();
+  G.row.resize(m);
+  if (a) G.a.resize(m);
+  if (b) G.b.resize(m);
+  std::vector<int32_t> fill(G.ptr.begin(), G.ptr.end() - 1);
+  for (size_t i = 0; i < m; ++i) {
+    const int32_t o = fill[rows[i] / rows_per_block]++;
+    G.row[o] = rows[i];
+    if (a) G.a[o] = (*a)[i];
+    if (b) G.b[o] = (*b)[i];
+  }
+  return G;
 }
 
 }  // namespace
@@ -257,11 +331,13 @@ struct bfe_plan_s {
   int64_t n_slices = 0, nnz_sell = 0, nnz_k = 0;
   int32_t nI = 0;
   int64_t nnz_coff = 0, nnz_xinv = 0;
-  double sq = 0.0, sr = 0.0;
   bool has_xinv = false;
+  double sq = 0.0, sr = 0.0;
   std::vector<double> alpha;
-  std::vector<int32_t> iface_pos;
-  DevBuf off, col, val, beta, gamma, iface, cptr, ccol, cval, xptr, xcol, xval;
+  DevBuf off, col, val, beta;
+  DevBuf iface, cptr, ccol, cval, gfix, xptr, xcol, xval;
+  // interface-row lists grouped per workgroup (single- and multi-source)
+  DevBuf is_ptr, is_row, is_k, im_ptr, im_row, im_k;
   bfe_run_stats stats{};
   int32_t timing_every = 0;
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
@@ -270,13 +346,20 @@ struct bfe_plan_s {
       (void)hipSetDevice(device);
       (void)hipStreamSynchronize(stream);
     }
-    off.release(); col.release(); val.release(); beta.release(); gamma.release();
-    iface.release(); cptr.release(); ccol.release(); cval.release();
-    xptr.release(); xcol.release(); xval.release();
+    for (DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr, &xcol,
+                      &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
+      b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     for (hipEvent_t e : tev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  int64_t device_bytes() const {
+    int64_t s = 0;
+    for (const DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr,
+                            &xcol, &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
+      s += static_cast<int64_t>(b->bytes);
+    return s;
   }
 };
 
@@ -347,92 +430,31 @@ struct KernelSampler {
 
 template <typename T>
 void upload_plan(bfe_plan_s& H, const bfe::Plan& P) {
-  hipStream_t st = H.stream;
-  H.off = upload(P.op.off, st);
-  H.col = upload(P.op.col, st);
-  H.val = upload(cast_vec<T>(P.op.val), st);
-  if (P.damped) {
-    H.beta = upload(cast_vec<T>(P.beta), st);
-    H.gamma = upload(cast_vec<T>(P.gamma), st);
-  }
+  H.off = upload(P.op.off);
+  H.col = upload(P.op.col);
+  H.val = upload(cast_vec<T>(P.op.val));
+  if (P.damped) H.beta = upload(cast_vec<T>(P.beta));
+  const int64_t n = P.n;
+  const int64_t nb_s = (n + kRowsSingle - 1) / kRowsSingle;
+  const int64_t nb_m = (n + kRowsMulti - 1) / kRowsMulti;
+  std::vector<int32_t> ks(P.iface.size());
+  for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
+  Grouped gs = group_by_block(nb_s, kRowsSingle, P.iface, &ks, nullptr);
+  Grouped gm = group_by_block(nb_m, kRowsMulti, P.iface, &ks, nullptr);
+  H.is_ptr = upload(gs.ptr); H.is_row = upload(gs.row); H.is_k = upload(gs.a);
+  H.im_ptr = upload(gm.ptr); H.im_row = upload(gm.row); H.im_k = upload(gm.a);
   if (!P.iface.empty()) {
-    H.iface = upload(P.iface, st);
-    H.cptr = upload(P.coff.ptr, st);
-    H.ccol = upload(P.coff.col, st);
-    H.cval = upload(cast_vec<T>(P.coff.val), st);
+    H.iface = upload(P.iface);
+    H.cptr = upload(P.coff.ptr);
+    H.ccol = upload(P.coff.col);
+    H.cval = upload(cast_vec<T>(P.coff.val));
+    H.gfix = upload(cast_vec<T>(P.gfix));
     if (P.has_xinv) {
-      H.xptr = upload(P.xinv.ptr, st);
-      H.xcol = upload(P.xinv.col, st);
-      H.xval = upload(cast_vec<T>(P.xinv.val), st);
+      H.xptr = upload(P.xinv.ptr);
+      H.xcol = upload(P.xinv.col);
+      H.xval = upload(cast_vec<T>(P.xinv.val));
     }
   }
-  BFE_HIP(hipStreamSynchronize(st));
-}
-
-// Host-side lists consumed by the step kernels.
-struct RunLists {
-  // forcing on non-interface rows, grouped by logical block
-  std::vector<int32_t> fptr, f_row, f_src, f_frow;
-  std::vector<double> f_alpha;
-  // forcing on interface rows
-  std::vector<int32_t> fi_k, fi_src, fi_frow;
-  std::vector<double> fi_alpha;
-  // monitors grouped by logical block
-  std::vector<int32_t> mptr, m_row, m_k;
-};
-
-// Forcing entries: (dof, source column, signal row). For one source column
-// the last entry for a DOF wins, like MATLAB's f(idx) = v.
-RunLists make_lists(const bfe_plan_s& H, int64_t rows_per_block, int64_t nblk,
-                    const std::vector<int32_t>& fdof, const std::vector<int32_t>& fsrc,
-                    const std::vector<int32_t>& frow, const int32_t* hist_idx, int32_t q) {
-  RunLists L;
-  // dedupe (dof, src) keeping the last occurrence
-  std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> key;
-  key.reserve(fdof.size());
-  for (size_t i = 0; i < fdof.size(); ++i) key.push_back({{fdof[i], fsrc[i]}, static_cast<int32_t>(i)});
-  std::stable_sort(key.begin(), key.end(),
-                   [](auto& a, auto& b) { return a.first < b.first; });
-  std::vector<int32_t> keep;
-  for (size_t i = 0; i < key.size(); ++i)
-    if (i + 1 == key.size() || key[i + 1].first != key[i].first) keep.push_back(key[i].second);
-  std::vector<std::vector<int32_t>> per_block_f;
-  std::vector<int64_t> fcount(nblk + 1, 0);
-  std::vector<int32_t> blk_entries;
-  for (int32_t i : keep) {
-    const int32_t d = fdof[i];
-    if (H.nI > 0 && H.iface_pos[d] >= 0) {
-      L.fi_k.push_back(H.iface_pos[d]);
-      L.fi_src.push_back(fsrc[i]);
-      L.fi_frow.push_back(frow[i]);
-      L.fi_alpha.push_back(H.alpha[d]);
-    } else {
-      blk_entries.push_back(i);
-      fcount[d / rows_per_block + 1]++;
-    }
-  }
-  L.fptr.assign(nblk + 1, 0);
-  for (int64_t b = 0; b < nblk; ++b) L.fptr[b + 1] = static_cast<int32_t>(L.fptr[b] + fcount[b + 1]);
-  const size_t nf = blk_entries.size();
-  L.f_row.resize(nf); L.f_src.resize(nf); L.f_frow.resize(nf); L.f_alpha.resize(nf);
-  std::vector<int32_t> fill(L.fptr.begin(), L.fptr.end() - 1);
-  for (int32_t i : blk_entries) {
-    const int32_t d = fdof[i];
-    const int32_t o = fill[d / rows_per_block]++;
-    L.f_row[o] = d; L.f_src[o] = fsrc[i]; L.f_frow[o] = frow[i]; L.f_alpha[o] = H.alpha[d];
-  }
-  // monitors
-  L.mptr.assign(nblk + 1, 0);
-  std::vector<int64_t> mcount(nblk + 1, 0);
-  for (int32_t k = 0; k < q; ++k) mcount[hist_idx[k] / rows_per_block + 1]++;
-  for (int64_t b = 0; b < nblk; ++b) L.mptr[b + 1] = static_cast<int32_t>(L.mptr[b] + mcount[b + 1]);
-  L.m_row.resize(q); L.m_k.resize(q);
-  std::vector<int32_t> mfill(L.mptr.begin(), L.mptr.end() - 1);
-  for (int32_t k = 0; k < q; ++k) {
-    const int32_t o = mfill[hist_idx[k] / rows_per_block]++;
-    L.m_row[o] = hist_idx[k]; L.m_k[o] = k;
-  }
-  return L;
 }
 
 // 0-based first column of F with a non-zero (or NaN) entry; -1 if none.
@@ -452,12 +474,85 @@ void check_indices(const int32_t* idx, int64_t count, int64_t n, const char* wha
       throw std::invalid_argument(std::string(what) + " index out of range");
 }
 
+// Everything a run needs on the device besides the plan.
+struct RunBuffers {
+  DevBuf F, fptr, frow, fsrc, ffrow, falpha, mptr, mrow, mk, hidx, wI;
+  BlockLists lists(const bfe_plan_s& H, bool multi) const {
+    BlockLists L;
+    L.fptr = fptr.as<int32_t>(); L.f_row = frow.as<int32_t>(); L.f_src = fsrc.as<int32_t>();
+    L.f_frow = ffrow.as<int32_t>();
+    L.mptr = mptr.as<int32_t>(); L.m_row = mrow.as<int32_t>(); L.m_k = mk.as<int32_t>();
+    L.iptr = (multi ? H.im_ptr : H.is_ptr).as<int32_t>();
+    L.i_row = (multi ? H.im_row : H.is_row).as<int32_t>();
+    L.i_k = (multi ? H.im_k : H.is_k).as<int32_t>();
+    return L;
+  }
+};
+
+// Builds forcing and monitor lists. Forcing entries are (dof, source
+// column, signal row); for one (dof, source) the last entry wins, like
+// MATLAB's f(idx) = v. Forcing on an interface row enters w_I with
+// alpha = dt^2/m, exactly as f enters y before X_II^-1 in v6:136.
 template <typename T>
-void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
-                const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
-                double* hist_out, int32_t field_every, int32_t field_is_vel,
-                double* field_out, int32_t* ti_start_out) {
-  const int64_t n = H.n;
+void make_run_buffers(RunBuffers& R, const bfe_plan_s& H, int64_t rows_per_block,
+                      int64_t nblk, int32_t S, const int32_t* frc_idx, const int32_t* frc_src,
+                      int32_t p, const double* frcs, int32_t frc_rows, int32_t n_t,
+                      const int32_t* hist_idx, int32_t q) {
+  std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> key;
+  for (int32_t i = 0; i < p; ++i) key.push_back({{frc_idx[i], frc_src ? frc_src[i] : 0}, i});
+  std::stable_sort(key.begin(), key.end(), [](auto& a, auto& b) { return a.first < b.first; });
+  std::vector<int32_t> rows, srcs, frow;
+  for (size_t i = 0; i < key.size(); ++i)
+    if (i + 1 == key.size() || key[i + 1].first != key[i].first) {
+      rows.push_back(key[i].first.first);
+      srcs.push_back(key[i].first.second);
+      frow.push_back(frc_rows == 1 ? 0 : key[i].second);
+    }
+  Grouped gf = group_by_block(nblk, rows_per_block, rows, &srcs, &frow);
+  std::vector<double> fa(gf.row.size());
+  for (size_t i = 0; i < fa.size(); ++i) fa[i] = H.alpha[gf.row[i]];
+  std::vector<int32_t> mrows(hist_idx, hist_idx + q), mks(q);
+  for (int32_t k = 0; k < q; ++k) mks[k] = k;
+  Grouped gmon = group_by_block(nblk, rows_per_block, mrows, &mks, nullptr);
+  std::vector<double> Fh(frcs, frcs + static_cast<size_t>(frc_rows) * n_t);
+  R.F = upload(cast_vec<T>(Fh));
+  R.fptr = upload(gf.ptr); R.frow = upload(gf.row); R.fsrc = upload(gf.a);
+  R.ffrow = upload(gf.b); R.falpha = upload(cast_vec<T>(fa));
+  R.mptr = upload(gmon.ptr); R.mrow = upload(gmon.row); R.mk = upload(gmon.a);
+  R.hidx = upload(mrows);
+  R.wI = DevBuf(sizeof(T) * static_cast<size_t>(H.nI) * S);
+}
+
+template <typename T>
+IfaceRows<T> iface_rows(const bfe_plan_s& H, T* wI) {
+  IfaceRows<T> I;
+  I.cptr = H.cptr.as<int64_t>();
+  I.ccol = H.ccol.as<int32_t>();
+  I.cval = H.cval.as<T>();
+  I.gfix = H.gfix.as<T>();
+  I.sq = static_cast<T>(H.sq);
+  I.sr = static_cast<T>(H.sr);
+  I.wI = H.has_xinv ? wI : nullptr;
+  return I;
+}
+
+template <typename T>
+void launch_iface(const bfe_plan_s& H, int32_t S, const T* wI, T* u) {
+  if (S == 1) {
+    const int64_t work = static_cast<int64_t>(H.nI) * kIfaceLanes;
+    hipLaunchKernelGGL((k_iface<T>), dim3((work + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       H.stream, H.nI, H.iface.as<int32_t>(), H.xptr.as<int64_t>(),
+                       H.xcol.as<int32_t>(), H.xval.as<T>(), wI, u);
+  } else {
+    const int64_t work = static_cast<int64_t>(H.nI) * S;
+    hipLaunchKernelGGL((k_iface_multi<T>), dim3((work + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       H.stream, H.nI, S, H.iface.as<int32_t>(), H.xptr.as<int64_t>(),
+                       H.xcol.as<int32_t>(), H.xval.as<T>(), wI, u);
+  }
+}
+
+void validate_run(int64_t n, int32_t n_t, const int32_t* frc_idx, int32_t p, const double* frcs,
+                  int32_t frc_rows, const int32_t* hist_idx, int32_t q, const double* hist_out) {
   if (n_t < 1) throw std::invalid_argument("n_t must be >= 1");
   if (p < 0 || q < 0) throw std::invalid_argument("negative count");
   check_indices(frc_idx, p, n, "forcing");
@@ -468,9 +563,41 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
       throw std::invalid_argument("forcing functions must have 1 or p rows");
   }
   if (q > 0 && hist_out == nullptr) throw std::invalid_argument("history output is NULL");
+}
+
+void finish_timing(bfe_plan_s& H, KernelSampler& sampler, int64_t steps, int64_t launches) {
+  BFE_HIP(hipGetLastError());
+  BFE_HIP(hipEventRecord(H.ev1, H.stream));
+  BFE_HIP(hipEventSynchronize(H.ev1));
+  float ms = 0.f;
+  BFE_HIP(hipEventElapsedTime(&ms, H.ev0, H.ev1));
+  H.stats.loop_ms = ms;
+  H.stats.steps = steps;
+  H.stats.launches = launches;
+  sampler.collect();
+}
+
+template <typename T>
+int64_t copy_out(const DevBuf& d, double* out, size_t count) {
+  std::vector<T> tmp(count);
+  BFE_HIP(hipMemcpy(tmp.data(), d.p, count * sizeof(T), hipMemcpyDeviceToHost));
+  int64_t bad = 0;
+  for (size_t i = 0; i < count; ++i) {
+    out[i] = static_cast<double>(tmp[i]);
+    bad += !std::isfinite(out[i]);
+  }
+  return bad;
+}
+
+template <typename T>
+void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
+                double* hist_out, int32_t field_every, int32_t field_is_vel,
+                double* field_out, int32_t* ti_start_out) {
+  const int64_t n = H.n;
+  validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   const int32_t n_frames = field_every > 0 ? (n_t - 1) / field_every + 1 : 0;
   if (n_frames > 0 && field_out == nullptr) throw std::invalid_argument("field output is NULL");
-
   if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * n_t);
   if (n_frames > 0) std::memset(field_out, 0, sizeof(double) * static_cast<size_t>(n) * n_frames);
   H.stats = bfe_run_stats{};
@@ -481,31 +608,24 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
-  const int64_t nblk = (n + kBlock - 1) / kBlock;
-  std::vector<int32_t> fdof(frc_idx, frc_idx + p), fsrc(p, 0), frow(p);
-  for (int32_t i = 0; i < p; ++i) frow[i] = frc_rows == 1 ? 0 : i;
-  RunLists L = make_lists(H, kBlock, nblk, fdof, fsrc, frow, hist_idx, q);
-
-  std::vector<double> Fh(frcs, frcs + static_cast<size_t>(p > 0 ? frc_rows : 0) * n_t);
-  DevBuf dF = upload(cast_vec<T>(Fh), st);
-  DevBuf dfptr = upload(L.fptr, st), dfrow = upload(L.f_row, st),
-         dffrow = upload(L.f_frow, st), dfalpha = upload(cast_vec<T>(L.f_alpha), st);
-  DevBuf dfik = upload(L.fi_k, st), dfisrc = upload(L.fi_src, st),
-         dfifrow = upload(L.fi_frow, st), dfialpha = upload(cast_vec<T>(L.fi_alpha), st);
-  DevBuf dmptr = upload(L.mptr, st), dmrow = upload(L.m_row, st), dmk = upload(L.m_k, st);
-  std::vector<int32_t> hidx(hist_idx, hist_idx + q);
-  DevBuf dhidx = upload(hidx, st);
+  const int64_t nblk = (n + kRowsSingle - 1) / kRowsSingle;
+  RunBuffers R;
+  make_run_buffers<T>(R, H, kRowsSingle, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
+                      hist_idx, q);
+  const BlockLists L = R.lists(H, false);
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * n_t);
   DevBuf dfld(sizeof(T) * static_cast<size_t>(n) * n_frames);
-  DevBuf dy(sizeof(T) * static_cast<size_t>(H.nI));
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * 3);
   BFE_HIP(hipMemsetAsync(ubuf.p, 0, ubuf.bytes, st));
   if (dhist.bytes) BFE_HIP(hipMemsetAsync(dhist.p, 0, dhist.bytes, st));
   if (dfld.bytes) BFE_HIP(hipMemsetAsync(dfld.p, 0, dfld.bytes, st));
-  T* U[3] = {ubuf.as<T>(), ubuf.as<T>() + n, ubuf.as<T>() + 2 * n};
-  T *u = U[0], *u1 = U[1], *u2 = U[2];
+  T *u = ubuf.as<T>(), *u1 = u + n, *u2 = u + 2 * n;
   T* hist = dhist.as<T>();
   T* fld = dfld.as<T>();
+  T* wI = R.wI.as<T>();
+  const IfaceRows<T> IR = iface_rows<T>(H, wI);
+  const T* falpha = R.falpha.as<T>();
+  const T* F = R.F.as<T>();
   const T dtT = static_cast<T>(H.dt);
   int64_t launches = 0;
   KernelSampler sampler(H, t0, n_t - t0);
@@ -516,33 +636,22 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     T* hp = (q > 0 && prev >= t0) ? hist + prev * q : nullptr;
     T* fp = (n_frames > 0 && prev >= t0 && prev % field_every == 0)
                 ? fld + (prev / field_every) * n : nullptr;
+    const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
     if (H.damped)
       hipLaunchKernelGGL((k_step<T, true>), dim3(nblk), dim3(kBlock), 0, st, n, nblk,
                          H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         H.beta.as<T>(), H.gamma.as<T>(), u1, u2, u, dfptr.as<int32_t>(),
-                         dfrow.as<int32_t>(), dfalpha.as<T>(), dffrow.as<int32_t>(),
-                         dF.as<T>(), frc_rows, static_cast<int32_t>(t), dmptr.as<int32_t>(),
-                         dmrow.as<int32_t>(), dmk.as<int32_t>(), hp, fp, field_is_vel, dtT);
+                         H.beta.as<T>(), u1, u2, u, L, falpha, F, frc_rows, ti, hp, fp,
+                         field_is_vel, dtT, IR);
     else
       hipLaunchKernelGGL((k_step<T, false>), dim3(nblk), dim3(kBlock), 0, st, n, nblk,
                          H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         static_cast<const T*>(nullptr), static_cast<const T*>(nullptr), u1,
-                         u2, u, dfptr.as<int32_t>(), dfrow.as<int32_t>(), dfalpha.as<T>(),
-                         dffrow.as<int32_t>(), dF.as<T>(), frc_rows, static_cast<int32_t>(t),
-                         dmptr.as<int32_t>(), dmrow.as<int32_t>(), dmk.as<int32_t>(), hp, fp,
-                         field_is_vel, dtT);
+                         static_cast<const T*>(nullptr), u1, u2, u, L, falpha, F, frc_rows, ti,
+                         hp, fp, field_is_vel, dtT, IR);
     sampler.after(t);
     ++launches;
-    if (H.nI > 0) {
-      hipLaunchKernelGGL((k_iface<T>), dim3(1), dim3(kIfaceThreads), 0, st, H.nI, 1,
-                         H.iface.as<int32_t>(), H.cptr.as<int64_t>(), H.ccol.as<int32_t>(),
-                         H.cval.as<T>(), static_cast<T>(H.sq), static_cast<T>(H.sr),
-                         H.has_xinv ? H.xptr.as<int64_t>() : nullptr, H.xcol.as<int32_t>(),
-                         H.xval.as<T>(), u1, u2, u, dy.as<T>(),
-                         static_cast<int32_t>(L.fi_k.size()), dfik.as<int32_t>(),
-                         dfisrc.as<int32_t>(), dfialpha.as<T>(), dfifrow.as<int32_t>(),
-                         dF.as<T>(), frc_rows, static_cast<int32_t>(t));
+    if (H.nI > 0 && H.has_xinv) {
+      launch_iface<T>(H, 1, wI, u);
       ++launches;
     }
     T* tmp = u2; u2 = u1; u1 = u; u = tmp;
@@ -554,36 +663,13 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     const int64_t work = std::max<int64_t>(fp ? n : 0, q);
     if (work > 0) {
       hipLaunchKernelGGL((k_snap<T>), dim3((work + 255) / 256), dim3(256), 0, st, n, 1, u1, u2,
-                         q, dhidx.as<int32_t>(), hp, fp, field_is_vel, dtT);
+                         q, R.hidx.as<int32_t>(), hp, fp, field_is_vel, dtT);
       ++launches;
     }
   }
-  BFE_HIP(hipGetLastError());
-  BFE_HIP(hipEventRecord(H.ev1, st));
-  BFE_HIP(hipEventSynchronize(H.ev1));
-  float ms = 0.f;
-  BFE_HIP(hipEventElapsedTime(&ms, H.ev0, H.ev1));
-  H.stats.loop_ms = ms;
-  H.stats.steps = n_t - t0;
-  H.stats.launches = launches;
-  sampler.collect();
-
-  std::vector<T> tmp;
-  if (q > 0) {
-    tmp.resize(static_cast<size_t>(q) * n_t);
-    BFE_HIP(hipMemcpy(tmp.data(), hist, dhist.bytes, hipMemcpyDeviceToHost));
-    int64_t bad = 0;
-    for (size_t i = 0; i < tmp.size(); ++i) {
-      hist_out[i] = static_cast<double>(tmp[i]);
-      bad += !std::isfinite(hist_out[i]);
-    }
-    H.stats.nonfinite = bad;
-  }
-  if (n_frames > 0) {
-    tmp.resize(static_cast<size_t>(n) * n_frames);
-    BFE_HIP(hipMemcpy(tmp.data(), fld, dfld.bytes, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < tmp.size(); ++i) field_out[i] = static_cast<double>(tmp[i]);
-  }
+  finish_timing(H, sampler, n_t - t0, launches);
+  if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(q) * n_t);
+  if (n_frames > 0) copy_out<T>(dfld, field_out, static_cast<size_t>(n) * n_frames);
 }
 
 template <typename T>
@@ -592,18 +678,9 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
                  const int32_t* hist_idx, int32_t q, double* hist_out,
                  int32_t* ti_start_out) {
   const int64_t n = H.n;
-  if (n_t < 1) throw std::invalid_argument("n_t must be >= 1");
+  validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   if (n_src < 1 || n_src > 64) throw std::invalid_argument("n_src must be in [1, 64]");
-  if (p < 0 || q < 0) throw std::invalid_argument("negative count");
-  check_indices(frc_idx, p, n, "forcing");
   check_indices(frc_src, p, n_src, "forcing source");
-  check_indices(hist_idx, q, n, "history");
-  if (p > 0) {
-    if (frcs == nullptr) throw std::invalid_argument("forcing functions are NULL");
-    if (frc_rows != 1 && frc_rows != p)
-      throw std::invalid_argument("forcing functions must have 1 or p rows");
-  }
-  if (q > 0 && hist_out == nullptr) throw std::invalid_argument("history output is NULL");
   const int32_t S = n_src;
   if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * S * n_t);
   H.stats = bfe_run_stats{};
@@ -614,30 +691,22 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
-  constexpr int64_t rows_per_block = kBlock / 64;
-  const int64_t nblk = (n + rows_per_block - 1) / rows_per_block;
-  std::vector<int32_t> fdof(frc_idx, frc_idx + p), fsrc(frc_src, frc_src + p), frow(p);
-  for (int32_t i = 0; i < p; ++i) frow[i] = frc_rows == 1 ? 0 : i;
-  RunLists L = make_lists(H, rows_per_block, nblk, fdof, fsrc, frow, hist_idx, q);
-
-  std::vector<double> Fh(frcs, frcs + static_cast<size_t>(p > 0 ? frc_rows : 0) * n_t);
-  DevBuf dF = upload(cast_vec<T>(Fh), st);
-  DevBuf dfptr = upload(L.fptr, st), dfrow = upload(L.f_row, st), dfsrc = upload(L.f_src, st),
-         dffrow = upload(L.f_frow, st), dfalpha = upload(cast_vec<T>(L.f_alpha), st);
-  DevBuf dfik = upload(L.fi_k, st), dfisrc = upload(L.fi_src, st),
-         dfifrow = upload(L.fi_frow, st), dfialpha = upload(cast_vec<T>(L.fi_alpha), st);
-  DevBuf dmptr = upload(L.mptr, st), dmrow = upload(L.m_row, st), dmk = upload(L.m_k, st);
-  std::vector<int32_t> hidx(hist_idx, hist_idx + q);
-  DevBuf dhidx = upload(hidx, st);
+  const int64_t nblk = (n + kRowsMulti - 1) / kRowsMulti;
+  RunBuffers R;
+  make_run_buffers<T>(R, H, kRowsMulti, nblk, S, frc_idx, frc_src, p, frcs, frc_rows, n_t,
+                      hist_idx, q);
+  const BlockLists L = R.lists(H, true);
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * S * n_t);
-  DevBuf dy(sizeof(T) * static_cast<size_t>(H.nI) * S);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * S * 3);
   BFE_HIP(hipMemsetAsync(ubuf.p, 0, ubuf.bytes, st));
   if (dhist.bytes) BFE_HIP(hipMemsetAsync(dhist.p, 0, dhist.bytes, st));
   const int64_t ns = n * S;
-  T* U[3] = {ubuf.as<T>(), ubuf.as<T>() + ns, ubuf.as<T>() + 2 * ns};
-  T *u = U[0], *u1 = U[1], *u2 = U[2];
+  T *u = ubuf.as<T>(), *u1 = u + ns, *u2 = u + 2 * ns;
   T* hist = dhist.as<T>();
+  T* wI = R.wI.as<T>();
+  const IfaceRows<T> IR = iface_rows<T>(H, wI);
+  const T* falpha = R.falpha.as<T>();
+  const T* F = R.F.as<T>();
   const int64_t per_t = static_cast<int64_t>(q) * S;
   int64_t launches = 0;
   KernelSampler sampler(H, t0, n_t - t0);
@@ -646,63 +715,33 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   for (int64_t t = t0; t < n_t; ++t) {
     const int64_t prev = t - 1;
     T* hp = (q > 0 && prev >= t0) ? hist + prev * per_t : nullptr;
+    const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
     if (H.damped)
       hipLaunchKernelGGL((k_step_multi<T, true>), dim3(nblk), dim3(kBlock), 0, st, n, S, nblk,
                          H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         H.beta.as<T>(), H.gamma.as<T>(), u1, u2, u, dfptr.as<int32_t>(),
-                         dfrow.as<int32_t>(), dfsrc.as<int32_t>(), dfalpha.as<T>(),
-                         dffrow.as<int32_t>(), dF.as<T>(), frc_rows, static_cast<int32_t>(t),
-                         dmptr.as<int32_t>(), dmrow.as<int32_t>(), dmk.as<int32_t>(), q, hp);
+                         H.beta.as<T>(), u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
     else
       hipLaunchKernelGGL((k_step_multi<T, false>), dim3(nblk), dim3(kBlock), 0, st, n, S, nblk,
                          H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         static_cast<const T*>(nullptr), static_cast<const T*>(nullptr), u1,
-                         u2, u, dfptr.as<int32_t>(), dfrow.as<int32_t>(), dfsrc.as<int32_t>(),
-                         dfalpha.as<T>(), dffrow.as<int32_t>(), dF.as<T>(), frc_rows,
-                         static_cast<int32_t>(t), dmptr.as<int32_t>(), dmrow.as<int32_t>(),
-                         dmk.as<int32_t>(), q, hp);
+                         static_cast<const T*>(nullptr), u1, u2, u, L, falpha, F, frc_rows, ti,
+                         q, hp, IR);
     sampler.after(t);
     ++launches;
-    if (H.nI > 0) {
-      hipLaunchKernelGGL((k_iface<T>), dim3(1), dim3(kIfaceThreads), 0, st, H.nI, S,
-                         H.iface.as<int32_t>(), H.cptr.as<int64_t>(), H.ccol.as<int32_t>(),
-                         H.cval.as<T>(), static_cast<T>(H.sq), static_cast<T>(H.sr),
-                         H.has_xinv ? H.xptr.as<int64_t>() : nullptr, H.xcol.as<int32_t>(),
-                         H.xval.as<T>(), u1, u2, u, dy.as<T>(),
-                         static_cast<int32_t>(L.fi_k.size()), dfik.as<int32_t>(),
-                         dfisrc.as<int32_t>(), dfialpha.as<T>(), dfifrow.as<int32_t>(),
-                         dF.as<T>(), frc_rows, static_cast<int32_t>(t));
+    if (H.nI > 0 && H.has_xinv) {
+      launch_iface<T>(H, S, wI, u);
       ++launches;
     }
     T* tmp = u2; u2 = u1; u1 = u; u = tmp;
   }
   if (q > 0) {
-    const int64_t work = per_t;
-    hipLaunchKernelGGL((k_snap<T>), dim3((work + 255) / 256), dim3(256), 0, st, n, S, u1, u2, q,
-                       dhidx.as<int32_t>(), hist + (n_t - 1) * per_t, static_cast<T*>(nullptr),
+    hipLaunchKernelGGL((k_snap<T>), dim3((per_t + 255) / 256), dim3(256), 0, st, n, S, u1, u2, q,
+                       R.hidx.as<int32_t>(), hist + (n_t - 1) * per_t, static_cast<T*>(nullptr),
                        0, static_cast<T>(H.dt));
     ++launches;
   }
-  BFE_HIP(hipGetLastError());
-  BFE_HIP(hipEventRecord(H.ev1, st));
-  BFE_HIP(hipEventSynchronize(H.ev1));
-  float ms = 0.f;
-  BFE_HIP(hipEventElapsedTime(&ms, H.ev0, H.ev1));
-  H.stats.loop_ms = ms;
-  H.stats.steps = n_t - t0;
-  H.stats.launches = launches;
-  sampler.collect();
-  if (q > 0) {
-    std::vector<T> tmp(static_cast<size_t>(per_t) * n_t);
-    BFE_HIP(hipMemcpy(tmp.data(), hist, dhist.bytes, hipMemcpyDeviceToHost));
-    int64_t bad = 0;
-    for (size_t i = 0; i < tmp.size(); ++i) {
-      hist_out[i] = static_cast<double>(tmp[i]);
-      bad += !std::isfinite(hist_out[i]);
-    }
-    H.stats.nonfinite = bad;
-  }
+  finish_timing(H, sampler, n_t - t0, launches);
+  if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(per_t) * n_t);
 }
 
 }  // namespace
@@ -747,11 +786,10 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->nI = static_cast<int32_t>(P.iface.size());
     H->nnz_coff = P.coff.nnz();
     H->nnz_xinv = P.has_xinv ? P.xinv.nnz() : 0;
+    H->has_xinv = P.has_xinv;
     H->sq = P.sq;
     H->sr = P.sr;
-    H->has_xinv = P.has_xinv;
     H->alpha = P.alpha;
-    H->iface_pos = P.iface_pos;
     if (precision == BFE_PREC_DOUBLE) upload_plan<double>(*H, P);
     else upload_plan<float>(*H, P);
     *out = H.release();
@@ -809,10 +847,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
     I.mode = h->mode;
     I.precision = h->prec;
     I.device = h->device;
-    I.device_bytes = static_cast<int64_t>(
-        h->off.bytes + h->col.bytes + h->val.bytes + h->beta.bytes + h->gamma.bytes +
-        h->iface.bytes + h->cptr.bytes + h->ccol.bytes + h->cval.bytes + h->xptr.bytes +
-        h->xcol.bytes + h->xval.bytes);
+    I.device_bytes = h->device_bytes();
     *info = I;
   });
 }

@@ -341,7 +341,7 @@ Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C, double d
     P.alpha[i] = a;
     P.beta[i] = b;
     P.gamma[i] = g;
-    if (b != 2.0 || g != -1.0) damped = true;
+    if (b != 2.0) damped = true;
   }
   P.damped = damped;
 
@@ -420,6 +420,11 @@ Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C, double d
     } else {
       P.sq = -dt;
       P.sr = dt;
+    }
+    P.gfix.resize(nI);
+    for (int64_t k = 0; k < nI; ++k) {
+      const int64_t i = P.iface[k];
+      P.gfix[k] = P.gamma[i] - (1.0 - P.beta[i]);
     }
   }
   return P;

@@ -44,7 +44,7 @@ struct Plan {
   int mode = 0;
   double dt = 0.0;
   std::vector<double> alpha, beta, gamma;
-  bool damped = false;
+  bool damped = false;  // some beta_i != 2 (gamma_i = 1 - beta_i is derived)
   int64_t nnz_k = 0;
   Sell64 op;  // alpha-scaled K
   // Interface rows (sorted) and their off-diagonal C scaled by 1/m_i.
@@ -54,6 +54,10 @@ struct Plan {
   double sq = 0.0, sr = 0.0;          // y += sq*Coff*u1 + sr*Coff*u2
   Csr xinv;                           // rows/cols over interface positions
   bool has_xinv = false;
+  // gamma_i = 1 - beta_i on every non-interface row (both modes) and on
+  // interface rows of the last-half-step mode; gfix_k = gamma - (1 - beta)
+  // on interface row k (dt c_i / (2 m_i) in the current-step mode).
+  std::vector<double> gfix;
 };
 
 // Normalises a CSR (sorted columns, duplicates summed, exact zeros removed)
