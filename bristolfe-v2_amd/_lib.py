@@ -20,7 +20,8 @@ PREC_SINGLE = 1
 
 # Every symbol the header declares.
 EXPORTS = (
-    "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_kernel_timing", "bfe_info_get",
+    "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
+    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_kernel_timing", "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error",
 )
 
@@ -77,6 +78,14 @@ def load() -> C.CDLL:
     lib.bfe_run_batched.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _i32p, C.c_int32,
                                     _f64p, C.c_int32, _i32p, C.c_int32, _f64p, _i32p]
     lib.bfe_run_batched.restype = C.c_int
+    lib.bfe_set_gradient_elements.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, _f64p,
+                                              _f64p, _f64p]
+    lib.bfe_set_gradient_elements.restype = C.c_int
+    lib.bfe_run_checkpointed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int32, _f64p,
+                                         C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32, _i32p]
+    lib.bfe_run_checkpointed.restype = C.c_int
+    lib.bfe_adjoint.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
+    lib.bfe_adjoint.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -191,6 +200,43 @@ class Plan:
             int(field_every if n_frames else 0), int(field_is_vel),
             fld.ctypes.data_as(_f64p) if fld is not None else None, C.byref(ti)))
         return hist, fld, int(ti.value)
+
+    def set_gradient_elements(self, el_dofs, el_K, el_m, el_mu):
+        """el_dofs (n_el, k) 0-based (-1 unused), el_K (n_el, k, k), el_m (n_el, k), el_mu (n_el)."""
+        el_dofs = np.ascontiguousarray(el_dofs, np.int32)
+        n_el, k = el_dofs.shape
+        el_K = np.ascontiguousarray(el_K, np.float64).reshape(n_el, k, k)
+        el_m = np.ascontiguousarray(el_m, np.float64).reshape(n_el, k)
+        el_mu = np.ascontiguousarray(el_mu, np.float64).reshape(n_el)
+        self.n_el = n_el
+        check(self._lib.bfe_set_gradient_elements(self._h, n_el, k, ptr(el_dofs, C.c_int32),
+                                                  ptr(el_K, C.c_double), ptr(el_m, C.c_double),
+                                                  ptr(el_mu, C.c_double)))
+
+    def run_checkpointed(self, slot, n_t, frc_idx, frcs, hist_idx, ckpt_every):
+        frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
+        hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
+        p, q = frc_idx.size, hist_idx.size
+        frcs = np.asarray(frcs, np.float64)
+        if frcs.ndim == 1:
+            frcs = frcs[None, :]
+        rows = frcs.shape[0] if p > 0 else 0
+        Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
+        hist = np.zeros((q, n_t), order="F")
+        ti = C.c_int32()
+        check(self._lib.bfe_run_checkpointed(
+            self._h, int(slot), int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p),
+            rows, ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(ckpt_every),
+            C.byref(ti)))
+        return hist, int(ti.value)
+
+    def adjoint(self, slot, hist_grad):
+        g = np.asfortranarray(hist_grad, dtype=np.float64)
+        gk = np.zeros(self.n_el)
+        gm = np.zeros(self.n_el)
+        check(self._lib.bfe_adjoint(self._h, int(slot), g.ctypes.data_as(_f64p),
+                                    gk.ctypes.data_as(_f64p), gm.ctypes.data_as(_f64p)))
+        return gk, gm
 
     def run_batched(self, n_t, n_src, frc_idx, frc_src, frcs, hist_idx):
         """Returns hist of shape (q, n_src, n_t) and ti_start."""

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -89,19 +90,26 @@ struct IfaceRows {
 };
 
 // ---------------------------------------------------------------------------
-// Single right-hand side step.
-//   u[r] = beta[r] u1[r] + (1 - beta[r]) u2[r] - sum_k val[k] u1[col[k]]  (+ forcing)
+// Row-per-thread step over S columns (S = 1: the march; S = 2: the adjoint
+// sweep that carries a recomputed forward column and the adjoint column
+// through one read of the operator). u is row-major n x S.
+//   u[r,s] = beta[r] u1[r,s] + (1 - beta[r]) u2[r,s] - sum_k val[k] u1[col[k],s] (+ forcing)
 // (gamma = 1 - beta holds on every row outside the interface; interface rows
 // add their correction from IfaceRows). DAMPED = false: beta = 2 everywhere
-// and no beta array is read.
-template <typename T, bool DAMPED>
+// and no beta array is read. store.p[s], when set, receives column s of u.
+template <typename T, int S>
+struct ColPtrs {
+  T* p[S];
+};
+
+template <typename T, int S, bool DAMPED>
 __global__ __launch_bounds__(kBlock) void k_step(
     int64_t n, int64_t nblk, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
     BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
-    int32_t t, T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    IfaceRows<T> I) {
+    int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
+    int32_t fld_is_vel, T dt, IfaceRows<T> I, ColPtrs<T, S> store) {
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t row = lb * kRowsSingle + threadIdx.x;
   if (row >= n) return;
@@ -109,38 +117,68 @@ __global__ __launch_bounds__(kBlock) void k_step(
   const int lane = static_cast<int>(row & 63);
   const int64_t e = off[sl + 1];
   int64_t p = off[sl] + lane;
-  T acc0 = T(0), acc1 = T(0);
+  T acc[S], acc2[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s] = acc2[s] = T(0);
   for (; p + 64 < e; p += 128) {
-    acc0 += val[p] * u1[col[p]];
-    acc1 += val[p + 64] * u1[col[p + 64]];
+    const T v0 = val[p], v1 = val[p + 64];
+    const int64_t c0 = static_cast<int64_t>(col[p]) * S, c1 = static_cast<int64_t>(col[p + 64]) * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      acc[s] += v0 * u1[c0 + s];
+      acc2[s] += v1 * u1[c1 + s];
+    }
   }
-  if (p < e) acc0 += val[p] * u1[col[p]];
-  const T a = u1[row], b = u2[row];
-  T un;
-  if constexpr (DAMPED) {
-    const T be = beta[row];
-    un = be * a + (T(1) - be) * b - (acc0 + acc1);  // gamma = 1 - beta
-  } else {
-    un = (T(2) * a - b) - (acc0 + acc1);
+  if (p < e) {
+    const T v0 = val[p];
+    const int64_t c0 = static_cast<int64_t>(col[p]) * S;
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] += v0 * u1[c0 + s];
+  }
+  T a[S], b[S], un[S];
+  const T be = DAMPED ? beta[row] : T(2);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    a[s] = u1[row * S + s];
+    b[s] = u2[row * S + s];
+    if constexpr (DAMPED) un[s] = be * a[s] + (T(1) - be) * b[s] - (acc[s] + acc2[s]);
+    else un[s] = (T(2) * a[s] - b[s]) - (acc[s] + acc2[s]);
   }
   for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i)
-    if (L.f_row[i] == row) un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+    if (L.f_row[i] == row) {
+      const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+#pragma unroll
+      for (int s = 0; s < S; ++s)
+        if (L.f_src[i] == s) un[s] += f;
+    }
   for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i)
     if (L.i_row[i] == row) {
       const int32_t k = L.i_k[i];
-      T c1 = T(0), c2 = T(0);
-      for (int64_t q = I.cptr[k], qe = I.cptr[k + 1]; q < qe; ++q) {
-        c1 += I.cval[q] * u1[I.ccol[q]];
-        c2 += I.cval[q] * u2[I.ccol[q]];
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        T c1 = T(0), c2 = T(0);
+        for (int64_t j = I.cptr[k], je = I.cptr[k + 1]; j < je; ++j) {
+          const int64_t c = static_cast<int64_t>(I.ccol[j]) * S + s;
+          c1 += I.cval[j] * u1[c];
+          c2 += I.cval[j] * u2[c];
+        }
+        un[s] += I.gfix[k] * b[s] + I.sq * c1 + I.sr * c2;
+        if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + s] = un[s];
       }
-      un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
-      if (I.wI != nullptr) I.wI[k] = un;
     }
-  u[row] = un;
-  if (fld_prev != nullptr) fld_prev[row] = fld_is_vel ? (a - b) / dt : a;
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    u[row * S + s] = un[s];
+    if (store.p[s] != nullptr) store.p[s][row] = un[s];
+  }
+  if constexpr (S == 1)
+    if (fld_prev != nullptr) fld_prev[row] = fld_is_vel ? (a[0] - b[0]) / dt : a[0];
   if (hist_prev != nullptr)
     for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i)
-      if (L.m_row[i] == row) hist_prev[L.m_k[i]] = a;
+      if (L.m_row[i] == row) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * s] = a[s];
+      }
 }
 
 // ---------------------------------------------------------------------------
@@ -249,6 +287,68 @@ __global__ void k_snap(int64_t n, int32_t S, const T* __restrict__ u_last,
 }
 
 // ---------------------------------------------------------------------------
+// Per-element gradients over one checkpoint segment of Ls time steps.
+// Thread e keeps K_e (KMAX x KMAX), the lumped mass per unit density and the
+// last two forward states of its DOFs in registers and walks the segment:
+//   gK_e -= dt^2 * sum_t lam_t' K_e u_{t-1}
+//   gM_e -= sum_t lam_t' Mbar_e (w0 u_t + w1 u_{t-1} + w2 u_{t-2})
+// U and Lam hold the segment's forward and adjoint states (Ls x n, column
+// t - segment start); um1/um2 are the two states before the segment.
+template <typename T, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_grad(
+    int32_t n_el, const int32_t* __restrict__ el_dofs, const T* __restrict__ el_K,
+    const T* __restrict__ el_m, const T* __restrict__ el_w, int64_t n,
+    const T* __restrict__ U, const T* __restrict__ Lam, int32_t Ls,
+    const T* __restrict__ um1, const T* __restrict__ um2, double dt2,
+    double* __restrict__ gK, double* __restrict__ gM) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_el) return;
+  int32_t d[KMAX];
+  T Ke[KMAX * KMAX], me[KMAX], p1[KMAX], p2[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) {
+    d[i] = el_dofs[e * KMAX + i];
+    me[i] = el_m[e * KMAX + i];
+    const int64_t di = d[i] >= 0 ? d[i] : 0;  // unused slots read DOF 0, then discard it
+    const T a1 = um1[di], a2 = um2[di];
+    p1[i] = d[i] >= 0 ? a1 : T(0);
+    p2[i] = d[i] >= 0 ? a2 : T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < KMAX * KMAX; ++i) Ke[i] = el_K[e * KMAX * KMAX + i];
+  const T w0 = el_w[3 * e], w1 = el_w[3 * e + 1], w2 = el_w[3 * e + 2];
+  double sk = 0.0, sm = 0.0;
+  for (int32_t t = 0; t < Ls; ++t) {
+    T ut[KMAX], lt[KMAX];
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      const int64_t o = static_cast<int64_t>(t) * n + (d[i] >= 0 ? d[i] : 0);
+      const T x = U[o], y = Lam[o];
+      ut[i] = d[i] >= 0 ? x : T(0);
+      lt[i] = d[i] >= 0 ? y : T(0);
+    }
+    T ks = T(0), ms = T(0);
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      T r = T(0);
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j) r += Ke[i * KMAX + j] * p1[j];
+      ks += lt[i] * r;
+      ms += lt[i] * me[i] * (w0 * ut[i] + w1 * p1[i] + w2 * p2[i]);
+    }
+    sk += static_cast<double>(ks);
+    sm += static_cast<double>(ms);
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      p2[i] = p1[i];
+      p1[i] = ut[i];
+    }
+  }
+  gK[e] -= dt2 * sk;
+  gM[e] -= sm;
+}
+
+// ---------------------------------------------------------------------------
 // Device buffers owned by a handle or a run.
 struct DevBuf {
   void* p = nullptr;
@@ -341,11 +441,16 @@ struct bfe_plan_s {
   bfe_run_stats stats{};
   int32_t timing_every = 0;
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
+  // adjoint support (defined further down)
+  std::map<int32_t, std::shared_ptr<void>> adj_records;  // AdjointRecord per slot
+  std::shared_ptr<void> grad_elems;  // GradElements
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
       (void)hipStreamSynchronize(stream);
     }
+    adj_records.clear();
+    grad_elems.reset();
     for (DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr, &xcol,
                       &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
       b->release();
@@ -589,11 +694,74 @@ int64_t copy_out(const DevBuf& d, double* out, size_t count) {
   return bad;
 }
 
+// Checkpoint segments of the marched range [t0, n_t): counted back from the
+// end in steps of C, so every segment but the first has exactly C steps.
+struct Segments {
+  std::vector<int64_t> start;  // nseg + 1 boundaries, start[nseg] = n_t
+  int64_t C = 0;
+  int64_t count() const { return static_cast<int64_t>(start.size()) - 1; }
+  int64_t len(int64_t k) const { return start[k + 1] - start[k]; }
+};
+
+Segments make_segments(int64_t t0, int64_t n_t, int64_t C) {
+  Segments S;
+  S.C = C;
+  const int64_t L = n_t - t0;
+  const int64_t nseg = (L + C - 1) / C;
+  const int64_t first = L - (nseg - 1) * C;
+  S.start.push_back(t0);
+  for (int64_t k = 1; k <= nseg; ++k) S.start.push_back(t0 + first + (k - 1) * C);
+  return S;
+}
+
+// What a checkpointed forward leaves on the handle for bfe_adjoint.
+struct AdjointRecord {
+  bool valid = false;
+  int32_t n_t = 0, p = 0, frc_rows = 0, q = 0;
+  int64_t t0 = -1;
+  Segments seg;
+  std::vector<int32_t> frc_idx, hist_idx;
+  std::vector<double> F;  // frc_rows x n_t
+  DevBuf ckpt;            // per segment k: u(start_k - 1), u(start_k - 2)
+  DevBuf useg[2];         // C x n forward states of one segment each
+  int cur = 0;            // useg[cur] holds the last segment after the forward
+};
+
+template <typename T>
+void launch_step(const bfe_plan_s& H, int S, int64_t nblk, const T* u1, const T* u2, T* u,
+                 const BlockLists& L, const T* falpha, const T* F, int32_t F_rows, int32_t t,
+                 int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt, const IfaceRows<T>& IR,
+                 T* store0, T* store1) {
+  hipStream_t st = H.stream;
+  const T* beta = H.damped ? H.beta.as<T>() : nullptr;
+  if (S == 1) {
+    ColPtrs<T, 1> sp{{store0}};
+    if (H.damped)
+      hipLaunchKernelGGL((k_step<T, 1, true>), dim3(nblk), dim3(kBlock), 0, st, H.n, nblk,
+                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(), beta, u1, u2, u,
+                         L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, sp);
+    else
+      hipLaunchKernelGGL((k_step<T, 1, false>), dim3(nblk), dim3(kBlock), 0, st, H.n, nblk,
+                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(), beta, u1, u2, u,
+                         L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, sp);
+  } else {
+    ColPtrs<T, 2> sp{{store0, store1}};
+    if (H.damped)
+      hipLaunchKernelGGL((k_step<T, 2, true>), dim3(nblk), dim3(kBlock), 0, st, H.n, nblk,
+                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(), beta, u1, u2, u,
+                         L, falpha, F, F_rows, t, q, hp, static_cast<T*>(nullptr), 0, dt, IR, sp);
+    else
+      hipLaunchKernelGGL((k_step<T, 2, false>), dim3(nblk), dim3(kBlock), 0, st, H.n, nblk,
+                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(), beta, u1, u2, u,
+                         L, falpha, F, F_rows, t, q, hp, static_cast<T*>(nullptr), 0, dt, IR, sp);
+  }
+}
+
 template <typename T>
 void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                 const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
                 double* hist_out, int32_t field_every, int32_t field_is_vel,
-                double* field_out, int32_t* ti_start_out) {
+                double* field_out, int32_t* ti_start_out, AdjointRecord* rec, int32_t ckpt_every) {
   const int64_t n = H.n;
   validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   const int32_t n_frames = field_every > 0 ? (n_t - 1) / field_every + 1 : 0;
@@ -604,6 +772,18 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   H.stats.columns = 1;
   const int64_t t0 = p > 0 ? first_forced_step(frcs, frc_rows, n_t) : -1;
   if (ti_start_out) *ti_start_out = t0 < 0 ? n_t + 1 : static_cast<int32_t>(t0 + 1);
+  if (rec) {
+    *rec = AdjointRecord{};
+    rec->n_t = n_t;
+    rec->p = p;
+    rec->frc_rows = frc_rows;
+    rec->q = q;
+    rec->t0 = t0;
+    rec->frc_idx.assign(frc_idx, frc_idx + p);
+    rec->hist_idx.assign(hist_idx, hist_idx + q);
+    if (p > 0) rec->F.assign(frcs, frcs + static_cast<size_t>(frc_rows) * n_t);
+    rec->valid = true;
+  }
   if (t0 < 0) return;  // empty loop (v6:163-180): outputs stay zero
 
   BFE_HIP(hipSetDevice(H.device));
@@ -627,6 +807,14 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   const T* falpha = R.falpha.as<T>();
   const T* F = R.F.as<T>();
   const T dtT = static_cast<T>(H.dt);
+  int64_t next_seg = 0;
+  if (rec) {
+    rec->seg = make_segments(t0, n_t, ckpt_every);
+    const int64_t nseg = rec->seg.count();
+    rec->ckpt = DevBuf(sizeof(T) * static_cast<size_t>(n) * 2 * nseg);
+    for (auto& b : rec->useg) b = DevBuf(sizeof(T) * static_cast<size_t>(n) * rec->seg.C);
+    rec->cur = 0;
+  }
   int64_t launches = 0;
   KernelSampler sampler(H, t0, n_t - t0);
 
@@ -636,23 +824,28 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     T* hp = (q > 0 && prev >= t0) ? hist + prev * q : nullptr;
     T* fp = (n_frames > 0 && prev >= t0 && prev % field_every == 0)
                 ? fld + (prev / field_every) * n : nullptr;
-    const int32_t ti = static_cast<int32_t>(t);
+    T* store = nullptr;
+    if (rec) {
+      const Segments& sg = rec->seg;
+      if (next_seg < sg.count() && t == sg.start[next_seg]) {
+        // checkpoint: the two states the step at t starts from
+        T* ck = rec->ckpt.as<T>() + next_seg * 2 * n;
+        BFE_HIP(hipMemcpyAsync(ck, u1, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        BFE_HIP(hipMemcpyAsync(ck + n, u2, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
+        ++next_seg;
+      }
+      const int64_t last = sg.count() - 1;
+      if (t >= sg.start[last]) store = rec->useg[0].as<T>() + (t - sg.start[last]) * n;
+    }
     sampler.before(t);
-    if (H.damped)
-      hipLaunchKernelGGL((k_step<T, true>), dim3(nblk), dim3(kBlock), 0, st, n, nblk,
-                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         H.beta.as<T>(), u1, u2, u, L, falpha, F, frc_rows, ti, hp, fp,
-                         field_is_vel, dtT, IR);
-    else
-      hipLaunchKernelGGL((k_step<T, false>), dim3(nblk), dim3(kBlock), 0, st, n, nblk,
-                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         static_cast<const T*>(nullptr), u1, u2, u, L, falpha, F, frc_rows, ti,
-                         hp, fp, field_is_vel, dtT, IR);
+    launch_step<T>(H, 1, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t), q, hp,
+                   fp, field_is_vel, dtT, IR, store, nullptr);
     sampler.after(t);
     ++launches;
     if (H.nI > 0 && H.has_xinv) {
       launch_iface<T>(H, 1, wI, u);
       ++launches;
+      if (store) BFE_HIP(hipMemcpyAsync(store, u, sizeof(T) * n, hipMemcpyDeviceToDevice, st));
     }
     T* tmp = u2; u2 = u1; u1 = u; u = tmp;
   }
@@ -670,6 +863,159 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   finish_timing(H, sampler, n_t - t0, launches);
   if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(q) * n_t);
   if (n_frames > 0) copy_out<T>(dfld, field_out, static_cast<size_t>(n) * n_frames);
+}
+
+// Element data for the parameter gradients.
+struct GradElements {
+  int32_t n_el = 0, kmax = 0;
+  DevBuf dofs, K, m, w;
+};
+
+template <typename T>
+void launch_grad(const bfe_plan_s& H, const GradElements& G, const T* U, const T* Lam, int32_t Ls,
+                 const T* um1, const T* um2, double* gK, double* gM) {
+  const dim3 grid((G.n_el + kBlock - 1) / kBlock), blk(kBlock);
+  const double dt2 = H.dt * H.dt;
+  const int32_t* d = G.dofs.as<int32_t>();
+  const T *Kp = G.K.as<T>(), *mp = G.m.as<T>(), *wp = G.w.as<T>();
+#define BFE_GRAD(KM)                                                                       \
+  hipLaunchKernelGGL((k_grad<T, KM>), grid, blk, 0, H.stream, G.n_el, d, Kp, mp, wp, H.n, U, \
+                     Lam, Ls, um1, um2, dt2, gK, gM)
+  switch (G.kmax) {
+    case 2: BFE_GRAD(2); break;
+    case 3: BFE_GRAD(3); break;
+    case 4: BFE_GRAD(4); break;
+    case 6: BFE_GRAD(6); break;
+    case 8: BFE_GRAD(8); break;
+    case 9: BFE_GRAD(9); break;
+    case 12: BFE_GRAD(12); break;
+    default: throw std::invalid_argument("unsupported element DOF count for gradients");
+  }
+#undef BFE_GRAD
+}
+
+// Reverse sweep: for k = last .. 0, the adjoint over segment k runs in
+// column 1 of a two-column march while column 0 recomputes segment k-1 from
+// its checkpoint; then the gradient kernel consumes segment k.
+template <typename T>
+void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
+                 const double* hist_grad, double* g_stiff, double* g_dens) {
+  const int64_t n = H.n;
+  std::memset(g_stiff, 0, sizeof(double) * G.n_el);
+  std::memset(g_dens, 0, sizeof(double) * G.n_el);
+  H.stats = bfe_run_stats{};
+  H.stats.columns = 2;
+  if (rec.t0 < 0) return;
+  BFE_HIP(hipSetDevice(H.device));
+  hipStream_t st = H.stream;
+  const int32_t n_t = rec.n_t;
+  const Segments& sg = rec.seg;
+  const int64_t nseg = sg.count(), C = sg.C;
+  const double dt2 = H.dt * H.dt;
+
+  // adjoint forcing: receivers with duplicate DOFs summed, scaled by 1/dt^2
+  std::vector<int32_t> rdof;
+  std::vector<int32_t> rrow(rec.q);
+  {
+    std::vector<std::pair<int32_t, int32_t>> kk;
+    for (int32_t k = 0; k < rec.q; ++k) kk.push_back({rec.hist_idx[k], k});
+    std::sort(kk.begin(), kk.end());
+    for (auto& e : kk) {
+      if (rdof.empty() || rdof.back() != e.first) rdof.push_back(e.first);
+      rrow[e.second] = static_cast<int32_t>(rdof.size()) - 1;
+    }
+  }
+  const int32_t qa = static_cast<int32_t>(rdof.size());
+  std::vector<double> Fadj(static_cast<size_t>(qa) * n_t, 0.0);
+  for (int64_t t = 0; t < n_t; ++t)
+    for (int32_t k = 0; k < rec.q; ++k)
+      Fadj[t * qa + rrow[k]] += hist_grad[t * rec.q + k] / dt2;
+
+  // forcing entries of the two-column march: forward (col 0) then adjoint (col 1)
+  const int32_t pf = rec.p, frow_f = rec.frc_rows;
+  const int32_t rows_dual = (pf > 0 ? frow_f : 0) + qa;
+  std::vector<int32_t> fi, fs;
+  for (int32_t i = 0; i < pf; ++i) { fi.push_back(rec.frc_idx[i]); fs.push_back(0); }
+  for (int32_t k = 0; k < qa; ++k) { fi.push_back(rdof[k]); fs.push_back(1); }
+  // signal rows: forward entry i uses row (frc_rows == 1 ? 0 : i); adjoint k uses frow_f + k.
+  // make_run_buffers maps entry i -> row i unless rows == 1, so give it per-entry rows.
+  const int32_t ptot = static_cast<int32_t>(fi.size());
+  std::vector<double> Fdual(static_cast<size_t>(rows_dual) * C, 0.0);
+  std::vector<double> Fentry(static_cast<size_t>(ptot) * C, 0.0);  // one row per entry
+  const int64_t nblk = (n + kRowsSingle - 1) / kRowsSingle;
+  RunBuffers R;
+  make_run_buffers<T>(R, H, kRowsSingle, nblk, 2, fi.data(), fs.data(), ptot, Fentry.data(),
+                      ptot, static_cast<int32_t>(C), nullptr, 0);
+  const BlockLists L = R.lists(H, false);
+  T* wI = R.wI.as<T>();
+  const IfaceRows<T> IR = iface_rows<T>(H, wI);
+  const T* falpha = R.falpha.as<T>();
+  DevBuf lam(sizeof(T) * static_cast<size_t>(n) * C);
+  DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * 2 * 3);
+  DevBuf zero(sizeof(T) * static_cast<size_t>(n) * 2);
+  DevBuf dgK(sizeof(double) * G.n_el), dgM(sizeof(double) * G.n_el);
+  BFE_HIP(hipMemsetAsync(ubuf.p, 0, ubuf.bytes, st));
+  BFE_HIP(hipMemsetAsync(zero.p, 0, zero.bytes, st));
+  BFE_HIP(hipMemsetAsync(dgK.p, 0, dgK.bytes, st));
+  BFE_HIP(hipMemsetAsync(dgM.p, 0, dgM.bytes, st));
+  T *u = ubuf.as<T>(), *u1 = u + 2 * n, *u2 = u + 4 * n;
+  const T dtT = static_cast<T>(H.dt);
+  int64_t launches = 0;
+  std::vector<T> Fj(static_cast<size_t>(ptot) * C);
+  KernelSampler sampler(H, 0, 0);
+
+  BFE_HIP(hipEventRecord(H.ev0, st));
+  for (int64_t k = nseg - 1; k >= 0; --k) {
+    const int64_t La = sg.len(k);
+    const int64_t Lf = k >= 1 ? sg.len(k - 1) : 0;
+    const int64_t a_end = sg.start[k + 1];        // adjoint walks t = a_end-1 .. start[k]
+    const int64_t f_beg = k >= 1 ? sg.start[k - 1] : 0;
+    T* unext = rec.useg[1 - rec.cur].as<T>();
+    if (k >= 1) {  // forward column restarts from checkpoint k-1
+      const T* ck = rec.ckpt.as<T>() + (k - 1) * 2 * n;
+      BFE_HIP(hipMemcpy2DAsync(u1, 2 * sizeof(T), ck, sizeof(T), sizeof(T), n,
+                               hipMemcpyDeviceToDevice, st));
+      BFE_HIP(hipMemcpy2DAsync(u2, 2 * sizeof(T), ck + n, sizeof(T), sizeof(T), n,
+                               hipMemcpyDeviceToDevice, st));
+    }
+    const int64_t steps = std::max(La, Lf);
+    // signals of this sweep, one row per forcing entry, column = sweep step
+    for (int64_t j = 0; j < steps; ++j) {
+      for (int32_t i = 0; i < pf; ++i)
+        Fj[j * ptot + i] = (j < Lf) ? static_cast<T>(rec.F[(f_beg + j) * frow_f + (frow_f == 1 ? 0 : i)])
+                                    : T(0);
+      for (int32_t r = 0; r < qa; ++r)
+        Fj[j * ptot + pf + r] = (j < La) ? static_cast<T>(Fadj[(a_end - 1 - j) * qa + r]) : T(0);
+    }
+    BFE_HIP(hipMemcpyAsync(R.F.p, Fj.data(), sizeof(T) * static_cast<size_t>(ptot) * steps,
+                           hipMemcpyHostToDevice, st));
+    for (int64_t j = 0; j < steps; ++j) {
+      T* s0 = j < Lf ? unext + j * n : nullptr;
+      T* s1 = j < La ? lam.as<T>() + (La - 1 - j) * n : nullptr;
+      launch_step<T>(H, 2, nblk, u1, u2, u, L, falpha, R.F.as<T>(), ptot, static_cast<int32_t>(j),
+                     0, nullptr, nullptr, 0, dtT, IR, s0, s1);
+      ++launches;
+      if (H.nI > 0 && H.has_xinv) {
+        launch_iface<T>(H, 2, wI, u);
+        ++launches;
+        // the interface solve rewrote u on interface rows: refresh the stores
+        if (s0) BFE_HIP(hipMemcpy2DAsync(s0, sizeof(T), u, 2 * sizeof(T), sizeof(T), n,
+                                         hipMemcpyDeviceToDevice, st));
+        if (s1) BFE_HIP(hipMemcpy2DAsync(s1, sizeof(T), u + 1, 2 * sizeof(T), sizeof(T), n,
+                                         hipMemcpyDeviceToDevice, st));
+      }
+      T* tmp = u2; u2 = u1; u1 = u; u = tmp;
+    }
+    // gradient contributions of segment k (states before it: checkpoint k)
+    const T* ck = rec.ckpt.as<T>() + k * 2 * n;
+    launch_grad<T>(H, G, rec.useg[rec.cur].as<T>(), lam.as<T>(), static_cast<int32_t>(La), ck,
+                   ck + n, dgK.as<double>(), dgM.as<double>());
+    ++launches;
+    rec.cur = 1 - rec.cur;
+  }
+  finish_timing(H, sampler, n_t - rec.t0, launches);
+  BFE_HIP(hipMemcpy(g_stiff, dgK.p, dgK.bytes, hipMemcpyDeviceToHost));
+  BFE_HIP(hipMemcpy(g_dens, dgM.p, dgM.bytes, hipMemcpyDeviceToHost));
 }
 
 template <typename T>
@@ -804,10 +1150,90 @@ bfe_status bfe_run(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
     if (h == nullptr) throw std::invalid_argument("NULL handle");
     if (h->prec == BFE_PREC_DOUBLE)
       run_single<double>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out,
-                         field_every, field_is_vel, field_out, ti_start_out);
+                         field_every, field_is_vel, field_out, ti_start_out, nullptr, 0);
     else
       run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out,
-                        field_every, field_is_vel, field_out, ti_start_out);
+                        field_every, field_is_vel, field_out, ti_start_out, nullptr, 0);
+  });
+}
+
+bfe_status bfe_run_checkpointed(bfe_handle h, int32_t slot, int32_t n_t, const int32_t* frc_idx,
+                                int32_t p, const double* frcs, int32_t frc_rows,
+                                const int32_t* hist_idx, int32_t q, double* hist_out,
+                                int32_t ckpt_every, int32_t* ti_start_out) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (ckpt_every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
+    if (slot < 0) throw std::invalid_argument("slot must be >= 0");
+    auto rec = std::make_shared<AdjointRecord>();
+    h->adj_records.erase(slot);
+    if (h->prec == BFE_PREC_DOUBLE)
+      run_single<double>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, 0, 1,
+                         nullptr, ti_start_out, rec.get(), ckpt_every);
+    else
+      run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, 0, 1,
+                        nullptr, ti_start_out, rec.get(), ckpt_every);
+    h->adj_records[slot] = rec;
+  });
+}
+
+bfe_status bfe_set_gradient_elements(bfe_handle h, int32_t n_el, int32_t kmax,
+                                     const int32_t* el_dofs, const double* el_K,
+                                     const double* el_m, const double* el_mu) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (n_el < 1 || kmax < 1) throw std::invalid_argument("n_el and kmax must be positive");
+    if (!el_dofs || !el_K || !el_m || !el_mu) throw std::invalid_argument("NULL element array");
+    const size_t ne = static_cast<size_t>(n_el);
+    for (size_t i = 0; i < ne * kmax; ++i)
+      if (el_dofs[i] < -1 || el_dofs[i] >= h->n)
+        throw std::invalid_argument("element DOF index out of range");
+    BFE_HIP(hipSetDevice(h->device));
+    auto G = std::make_shared<GradElements>();
+    G->n_el = n_el;
+    G->kmax = kmax;
+    // mass-term weights of dR/drho_e per solver mode (see run_adjoint)
+    std::vector<double> w(3 * ne);
+    const double dt = h->dt;
+    for (size_t e = 0; e < ne; ++e) {
+      const double mu = el_mu[e];
+      if (h->mode == BFE_MODE_VEL_AT_CURRENT) {
+        w[3 * e] = 1.0 + 0.5 * dt * mu; w[3 * e + 1] = -2.0; w[3 * e + 2] = 1.0 - 0.5 * dt * mu;
+      } else {
+        w[3 * e] = 1.0; w[3 * e + 1] = -(2.0 - dt * mu); w[3 * e + 2] = 1.0 - dt * mu;
+      }
+    }
+    std::vector<int32_t> d(el_dofs, el_dofs + ne * kmax);
+    std::vector<double> K(el_K, el_K + ne * kmax * kmax), m(el_m, el_m + ne * kmax);
+    G->dofs = upload(d);
+    if (h->prec == BFE_PREC_DOUBLE) {
+      G->K = upload(K); G->m = upload(m); G->w = upload(w);
+    } else {
+      G->K = upload(cast_vec<float>(K)); G->m = upload(cast_vec<float>(m));
+      G->w = upload(cast_vec<float>(w));
+    }
+    h->grad_elems = G;
+  });
+}
+
+bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad, double* g_stiffness,
+                       double* g_density) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    auto it = h->adj_records.find(slot);
+    auto rec = it == h->adj_records.end() ? nullptr
+                                          : std::static_pointer_cast<AdjointRecord>(it->second);
+    auto G = std::static_pointer_cast<GradElements>(h->grad_elems);
+    if (!rec || !rec->valid)
+      throw std::invalid_argument("no checkpointed forward run to differentiate");
+    if (!G) throw std::invalid_argument("gradient elements not set");
+    if (!g_stiffness || !g_density) throw std::invalid_argument("NULL gradient output");
+    if (rec->q > 0 && hist_grad == nullptr) throw std::invalid_argument("NULL history gradient");
+    if (h->prec == BFE_PREC_DOUBLE)
+      run_adjoint<double>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+    else
+      run_adjoint<float>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+    h->adj_records.erase(slot);  // the forward states were consumed
   });
 }
 

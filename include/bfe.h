@@ -151,6 +151,48 @@ bfe_status bfe_run_batched(bfe_handle h, int32_t n_t, int32_t n_src,
                            int32_t* ti_start_out);
 
 /*
+ * Differentiation (no counterpart in the reference, which has no adjoint).
+ * Parameters: a stiffness scale s_e and a density rho_e per element, with
+ *   K = sum_e s_e K_e,  M = sum_e rho_e Mbar_e,  C_diag = sum_e mu_e rho_e Mbar_e
+ * (K_e including the absorbing-layer reduction of
+ * fn_build_global_matrices_v5.m:116, C from :115), evaluated at s_e = 1 and
+ * the model's densities. The adjoint is the same recurrence marched backward;
+ * gradients are accumulated in FP64.
+ *
+ * bfe_set_gradient_elements — element data of the parameters:
+ *   el_dofs (n_el x kmax, row-major) 0-based global DOFs, -1 for unused slots
+ *   el_K    (n_el x kmax x kmax)     K_e at s_e = 1
+ *   el_m    (n_el x kmax)            diagonal of dM/drho_e (lumped mass per unit density)
+ *   el_mu   (n_el)                   damping per unit mass, C_e = mu_e M_e
+ */
+bfe_status bfe_set_gradient_elements(bfe_handle h, int32_t n_el, int32_t kmax,
+                                     const int32_t* el_dofs, const double* el_K,
+                                     const double* el_m, const double* el_mu);
+
+/*
+ * bfe_run_checkpointed — bfe_run (no field output) that also keeps, on the
+ * device and under record `slot`, the two states at the start of every
+ * ckpt_every-step segment (counted back from the last step) and all states of
+ * the last segment, for one following bfe_adjoint on the same slot. Several
+ * slots (e.g. one per transmitter) may be held at once.
+ */
+bfe_status bfe_run_checkpointed(bfe_handle h, int32_t slot, int32_t n_t, const int32_t* frc_idx,
+                                int32_t p, const double* frcs, int32_t frc_rows,
+                                const int32_t* hist_idx, int32_t q, double* hist_out,
+                                int32_t ckpt_every, int32_t* ti_start_out);
+
+/*
+ * bfe_adjoint — gradients of a scalar J(hist) of the checkpointed run held
+ * under `slot` (which it consumes).
+ *   hist_grad (q x n_t)  dJ/dhist, the layout of hist_out
+ *   g_stiffness, g_density (n_el)  dJ/ds_e and dJ/drho_e
+ * The reverse sweep recomputes each segment from its checkpoint in the same
+ * launches that march the adjoint over the following segment.
+ */
+bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
+                       double* g_stiffness, double* g_density);
+
+/*
  * bfe_set_kernel_timing — sample the duration of the main step kernel.
  * No counterpart in the reference (its only timing is the wall clock of
  * v6:220). every > 0 brackets every every-th step-kernel launch with HIP
