@@ -107,6 +107,172 @@ def cpu_baseline(c, hi, budget_s=15.0):
                        f"(OpenMP, {threads} threads), loop time {el:.2f} s, setup {setup:.1f} s")}, hist, steps
 
 
+def bench_adjoint(args):
+    """Differentiable workload: gradient of the A-scan misfit w.r.t. per-element
+    stiffness scale and density; reports t(forward + adjoint incl. recompute and
+    gradient) / t(plain forward) and a central-difference check."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from bristolfe_v2_amd import adjoint as adj
+    from bristolfe_v2_amd import configs
+    kw = {} if args.time_steps is None else {"n_t": args.time_steps}
+    t0 = time.perf_counter()
+    c = configs.adjoint_plate(**kw)
+    pm, shots = c["model"], c["shots"]
+    log(f"workload {c['name']}: n={pm.n} elements={pm.n_el} setup {time.perf_counter() - t0:.1f} s")
+    data = adj.forward(pm, shots, c["s_true"], c["rho_true"])
+    s0, r0 = np.ones(pm.n_el), pm.rho0.copy()
+    # plain forward (no checkpoints), all transmitters
+    plan = pm.plan(shots.dt, s0, r0)
+    t_plain = []
+    for rep in range(args.warmup + args.steps):
+        ms = 0.0
+        for tx in shots.tx:
+            plan.run(shots.n_t, tx, shots.signal, shots.rx)
+            ms += plan.stats()["loop_ms"]
+        if rep >= args.warmup:
+            t_plain.append(ms)
+    plan.close()
+    t_fwd, t_adj = [], []
+    for rep in range(args.warmup + args.steps):
+        A, plan, ms_f = adj.forward(pm, shots, s0, r0, keep=True)
+        gk, gm, ms_a = adj.backward(plan, A - data)
+        plan.close()
+        if rep >= args.warmup:
+            t_fwd.append(ms_f)
+            t_adj.append(ms_a)
+        log(f"rep {rep}: forward(ckpt) {ms_f:.1f} ms, adjoint {ms_a:.1f} ms")
+    J0 = 0.5 * np.sum((A - data) ** 2)
+    tp, tf, ta = float(np.mean(t_plain)), float(np.mean(t_fwd)), float(np.mean(t_adj))
+    # central differences on seeded elements (GPU FP64 forward, all transmitters)
+    rng = np.random.default_rng(2024)
+    cand = np.nonzero(np.abs(gk) > 1e-3 * np.abs(gk).max())[0]
+    pick = rng.choice(cand, min(args.fd_elements, cand.size), replace=False)
+    errs_s, errs_r = [], []
+    for e in pick:
+        for which in ("s", "rho"):
+            base = s0 if which == "s" else r0
+            h = args.fd_step * abs(base[e])
+            vals = []
+            for sgn in (1, -1):
+                th = base.copy()
+                th[e] += sgn * h
+                Ah = adj.ascans_batched(pm, shots, th if which == "s" else s0,
+                                        th if which == "rho" else r0)
+                vals.append(0.5 * np.sum((Ah - data) ** 2))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            g = gk[e] if which == "s" else gm[e]
+            (errs_s if which == "s" else errs_r).append(abs(g - fd) / max(abs(fd), 1e-300))
+        log(f"fd element {e}: rel err s {errs_s[-1]:.2e} rho {errs_r[-1]:.2e}")
+    out = {
+        "metric": "adjoint/fwd cost", "value": round((tf + ta) / tp, 4), "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(tf + ta, 3), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": c["name"], "n_dof": int(pm.n), "parameters": 2 * int(pm.n_el),
+                   "transmitters": len(shots.tx), "time_steps": shots.n_t,
+                   "checkpoint_every": shots.ckpt_every,
+                   "step": "one gradient: forward of all transmitters + adjoint"},
+        "forward_plain_ms": round(tp, 3), "forward_checkpointed_ms": round(tf, 3),
+        "adjoint_ms": round(ta, 3), "misfit": J0,
+        "fd_check": {"elements": int(len(pick)), "step_rel": args.fd_step,
+                     "max_rel_err_stiffness": float(np.max(errs_s)) if errs_s else None,
+                     "max_rel_err_density": float(np.max(errs_r)) if errs_r else None,
+                     "median_rel_err": float(np.median(errs_s + errs_r)) if errs_s else None},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_fmc(args):
+    """Batched multi-source workload: 64 FMC transmitters on the plate plus 8
+    subdomain models in one 64-column march, against 64 single-source runs."""
+    import __graft_entry__ as ge
+    ge.load_package()
+    from bristolfe_v2_amd import autotune, configs
+    from bristolfe_v2_amd.solver import make_plan
+    from oracle import v6 as ov6
+    kw = {} if args.time_steps is None else {"n_t": args.time_steps}
+    t0 = time.perf_counter()
+    c = configs.fmc_multimodel(**kw)
+    K, C, M = c["K"], c["C"], c["M"]
+    n, S, n_t = K.shape[0], c["n_src"], len(c["time"])
+    fd, fs, hd, F = c["frc_dof"], c["frc_src"], c["hist_dof"], c["frcs"]
+    log(f"workload {c['name']}: n={n} nnz={K.nnz} models={c['model_sizes']} "
+        f"setup {time.perf_counter() - t0:.1f} s")
+    res = {}
+    for prec in ("double", "single"):
+        plan = make_plan(K, C, M, c["dt"], "vel at curent time step", prec)
+        best, table = autotune.tune_batched(plan, S, fd, fs, F, hd)
+        log(f"{prec}: autotune best {best}")
+        walls, loops = [], []
+        hb = None
+        for rep in range(args.warmup + args.steps):
+            w0 = time.perf_counter()
+            hb, ms = autotune.run_tiled(plan, n_t, S, fd, fs, F, hd, best["tile"])
+            w = time.perf_counter() - w0
+            if rep >= args.warmup:
+                walls.append(w)
+                loops.append(ms)
+        res[prec] = dict(plan=plan, best=best, table=table, wall=float(np.mean(walls)),
+                         loop_ms=float(np.mean(loops)), hist=hb)
+        log(f"{prec}: batched {S} columns: loop {np.mean(loops):.1f} ms, wall {np.mean(walls):.2f} s")
+    # 64 plain single-source launches (FP64), forcing upload and history download included
+    plan = res["double"]["plan"]
+    w0 = time.perf_counter()
+    plain_loop = 0.0
+    worst = 0.0
+    for s in range(S):
+        sel = fs == s
+        h, _, _ = plan.run(n_t, fd[sel], F, hd)
+        plain_loop += plan.stats()["loop_ms"]
+        b = res["double"]["hist"][:, s, :]
+        worst = max(worst, float(np.linalg.norm(h - b) / max(np.linalg.norm(h), 1e-300)))
+    plain_wall = time.perf_counter() - w0
+    log(f"plain: {S} runs wall {plain_wall:.2f} s loop {plain_loop:.1f} ms; worst column diff {worst:.2e}")
+    h64, h32 = res["double"]["hist"], res["single"]["hist"]
+    err32 = float(np.linalg.norm(h32 - h64) / np.linalg.norm(h64))
+    # oracle spot check on the plate block for two transmitters
+    n0 = c["model_sizes"][0]
+    K0, C0, M0 = K[:n0, :n0], C[:n0, :n0], M[:n0, :n0]
+    q0 = np.nonzero(hd < n0)[0]
+    orc = []
+    for s in (0, S // 2):
+        sel = (fs == s) & (fd < n0)
+        r, _, _, _ = ov6.explicit_dynamic_solver_v6(K0, C0, M0, c["time"], fd[sel] + 1, F, hd[q0] + 1)
+        orc.append(float(np.linalg.norm(h64[q0, s, :] - r) / np.linalg.norm(r)))
+    info = plan.info()
+    speedup = plain_wall / res["double"]["wall"]
+    bpd = (12.0 * info["nnz_k"] / n + 4.0) / S + 24.0 + (16.0 / S if info["damped"] else 0.0)
+    gdofs = n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9
+    out = {
+        "metric": "batched per-source speedup", "value": round(speedup, 3), "unit": "x",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(res["double"]["wall"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": c["name"], "n_dof": int(n), "models": c["model_sizes"],
+                   "columns": int(S), "time_steps": int(n_t),
+                   "step": "one 64-column march of all models (bfe_run_batched)"},
+        "batched_gdof_steps_per_s": round(gdofs, 3),
+        "roofline": {"bound": "hbm", "achieved": round(bpd * n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bpd * n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_step_multi<double> (64 sources, one wavefront per row)",
+                     "bytes_per_dof_step_source": round(bpd, 2)},
+        "plain_wall_s": round(plain_wall, 3), "batched_wall_s": round(res["double"]["wall"], 3),
+        "plain_loop_ms": round(plain_loop, 1), "batched_loop_ms": round(res["double"]["loop_ms"], 1),
+        "speedup_loop_only": round(plain_loop / res["double"]["loop_ms"], 3),
+        "batched_vs_single_max_rel_l2": worst,
+        "oracle_rel_l2_plate_block": orc,
+        "fp32": {"rel_l2_vs_fp64": err32, "wall_s": round(res["single"]["wall"], 3),
+                 "speedup_vs_fp64_batched": round(res["double"]["wall"] / res["single"]["wall"], 3),
+                 "autotune": res["single"]["best"]},
+        "autotune": {"chosen": res["double"]["best"], "candidates": res["double"]["table"]},
+    }
+    for r in res.values():
+        r["plan"].close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,9 +285,17 @@ def main():
                     help="override the workload's number of time steps (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device copy (counter calibration)")
+    ap.add_argument("--fd-elements", type=int, default=50)
+    ap.add_argument("--fd-step", type=float, default=1e-5)
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.gpus == 1 and args.workload == "adjoint_2d":
+        bench_adjoint(args)
+        return
+    if args.gpus == 1 and args.workload == "fmc_batched":
+        bench_fmc(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the march is single-GPU; --gpus {args.gpus} "

@@ -17,12 +17,16 @@ MODE_VEL_AT_CURRENT = 0
 MODE_VEL_AT_LAST_HALF = 1
 PREC_DOUBLE = 0
 PREC_SINGLE = 1
+TUNE_MULTI_ROWS_PER_BLOCK = 1
+TUNE_XCD_REMAP = 2
 
 # Every symbol the header declares.
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
-    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_kernel_timing", "bfe_info_get",
-    "bfe_last_run_stats", "bfe_destroy", "bfe_last_error",
+    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_kernel_timing",
+    "bfe_info_get",
+    "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
+    "bfe_assemble_fill", "bfe_assemble_last_error",
 )
 
 
@@ -86,6 +90,8 @@ def load() -> C.CDLL:
     lib.bfe_run_checkpointed.restype = C.c_int
     lib.bfe_adjoint.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
     lib.bfe_adjoint.restype = C.c_int
+    lib.bfe_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.bfe_set_tuning.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -96,6 +102,15 @@ def load() -> C.CDLL:
     lib.bfe_destroy.restype = C.c_int
     lib.bfe_last_error.argtypes = []
     lib.bfe_last_error.restype = C.c_char_p
+    lib.bfe_assemble_pattern.argtypes = [C.c_int64, C.c_int32, _f64p, C.c_int64, C.c_int32, _i32p,
+                                         _i32p, _i32p, _f64p, C.c_int32, _f64p, _f64p, C.c_int32,
+                                         C.c_double, C.c_double, C.c_double, _i64p, _i64p, _i32p,
+                                         _p(C.c_void_p)]
+    lib.bfe_assemble_pattern.restype = C.c_int
+    lib.bfe_assemble_fill.argtypes = [C.c_void_p, _i64p, _i32p, _f64p, _f64p, _f64p, _i64p]
+    lib.bfe_assemble_fill.restype = C.c_int
+    lib.bfe_assemble_last_error.argtypes = []
+    lib.bfe_assemble_last_error.restype = C.c_char_p
     _lib = lib
     return lib
 
@@ -171,6 +186,9 @@ class Plan:
         inf = BfeInfo()
         check(self._lib.bfe_info_get(self._h, C.byref(inf)))
         return {f: getattr(inf, f) for f, _ in BfeInfo._fields_}
+
+    def set_tuning(self, key: int, value: int) -> None:
+        check(self._lib.bfe_set_tuning(self._h, int(key), int(value)))
 
     def set_kernel_timing(self, every: int) -> None:
         check(self._lib.bfe_set_kernel_timing(self._h, int(every)))
@@ -256,3 +274,37 @@ class Plan:
             Fcol.ctypes.data_as(_f64p), rows, ptr(hist_idx, C.c_int32), q,
             hist.ctypes.data_as(_f64p), C.byref(ti)))
         return hist, int(ti.value)
+
+
+def assemble_native(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mask, p_law,
+                    max_damping, msr):
+    """Calls bfe_assemble_pattern/fill. 0-based element nodes and materials.
+    Returns (rowptr, col, kval, m, c, gl_lookup)."""
+    lib = load()
+    nds = np.ascontiguousarray(nds, np.float64)
+    els0 = np.ascontiguousarray(els0, np.int32)
+    el_type = np.ascontiguousarray(el_type, np.int32)
+    el_mat0 = np.ascontiguousarray(el_mat0, np.int32)
+    el_abs = np.ascontiguousarray(el_abs, np.float64)
+    mat_D = np.ascontiguousarray(mat_D, np.float64)
+    mat_rho = np.ascontiguousarray(mat_rho, np.float64)
+    n_dof, nnz, max_df, work = C.c_int64(), C.c_int64(), C.c_int32(), C.c_void_p()
+    st = lib.bfe_assemble_pattern(
+        nds.shape[0], nds.shape[1], ptr(nds, C.c_double), els0.shape[0], els0.shape[1],
+        ptr(els0, C.c_int32), ptr(el_type, C.c_int32), ptr(el_mat0, C.c_int32), ptr(el_abs, C.c_double),
+        mat_rho.size, ptr(mat_D, C.c_double), ptr(mat_rho, C.c_double), int(dof_mask), float(p_law),
+        float(max_damping), float(msr), C.byref(n_dof), C.byref(nnz), C.byref(max_df), C.byref(work))
+    if st != BFE_OK:
+        raise BfeError(st, lib.bfe_assemble_last_error().decode())
+    n, z = n_dof.value, nnz.value
+    rowptr = np.empty(n + 1, np.int64)
+    col = np.empty(z, np.int32)
+    kval = np.empty(z, np.float64)
+    m = np.empty(n, np.float64)
+    c = np.empty(n, np.float64)
+    gl = np.empty((nds.shape[0], max_df.value), np.int64)
+    st = lib.bfe_assemble_fill(work, ptr(rowptr, C.c_int64), ptr(col, C.c_int32), ptr(kval, C.c_double),
+                               ptr(m, C.c_double), ptr(c, C.c_double), ptr(gl, C.c_int64))
+    if st != BFE_OK:
+        raise BfeError(st, lib.bfe_assemble_last_error().decode())
+    return rowptr, col, kval, m, c, gl

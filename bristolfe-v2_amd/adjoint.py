@@ -171,6 +171,23 @@ def forward(model: ParametrizedModel, shots: Shots, s=None, rho=None, keep=False
     return out
 
 
+def ascans_batched(model: ParametrizedModel, shots: Shots, s=None, rho=None):
+    """A-scans (n_tx, n_rx, n_t) of all transmitters in one multi-source march
+    (no checkpoints): the cheap evaluator for finite differences."""
+    K, C, M = model.matrices(s, rho)
+    o = model.fe_options
+    out = np.zeros((len(shots.tx), shots.rx.size, shots.n_t))
+    for c0 in range(0, len(shots.tx), 64):
+        grp = shots.tx[c0:c0 + 64]
+        fi = np.concatenate(grp)
+        fs = np.concatenate([np.full(t.size, k) for k, t in enumerate(grp)])
+        with _lib.Plan.from_scipy(K, np.asarray(M.sum(axis=0)).ravel(), C, shots.dt,
+                                  SOLVER_MODES[o["solver_mode"]], _precision(o["solver_precision"])) as p:
+            h, _ = p.run_batched(shots.n_t, len(grp), fi, fs, shots.signal, shots.rx)
+        out[c0:c0 + len(grp)] = np.transpose(h, (1, 0, 2))
+    return out
+
+
 def backward(plan, grad_ascans):
     """dJ/ds, dJ/drho from dJ/dA-scans; consumes the plan's checkpoints."""
     gk = gm = None

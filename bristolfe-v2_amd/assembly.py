@@ -40,7 +40,13 @@ DEFAULTS = {
     "max_damping": 3.1415e7,
     "max_stiffness_reduction": 0.01,
     "interface_damping_factor": 0,
+    # "numpy": vectorised coordinate assembly (any element type);
+    # "native": libbfe_hip's row-by-row assembler (CPE3/AC2D3/C3D4, no
+    # interface elements); "auto": native for more than 2M such elements.
+    "assembly_backend": "auto",
 }
+
+_NATIVE_TYPES = {"CPE3": 0, "AC2D3": 1, "C3D4": 2}
 
 
 def find_dof_in_use_and_max_dof_per_el(unique_typs, dof_to_use=None):
@@ -82,6 +88,12 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
 
     dof_to_use = fe_options["dof_to_use"]
     unique_typs = sorted(set(el_typ_i.tolist()))
+    backend = fe_options["assembly_backend"]
+    native_ok = set(unique_typs) <= set(_NATIVE_TYPES) and not fe_options["interface_damping_factor"]
+    if backend == "native" and not native_ok:
+        raise ValueError("native assembly supports CPE3, AC2D3 and C3D4 elements only")
+    if backend == "native" or (backend == "auto" and native_ok and els.shape[0] > 2_000_000):
+        return _assemble_native(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_options)
     unique_df, _ = find_dof_in_use_and_max_dof_per_el(unique_typs, dof_to_use)
     max_df = int(unique_df.max())
     no_nds = nds.shape[0]
@@ -163,6 +175,38 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
         C = fn_add_interface_damping(els, el_typ_i, K, C, M, gl_lookup,
                                      fe_options["interface_damping_factor"])
     return K, C, M, gl_lookup
+
+
+def _assemble_native(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_options):
+    from . import _lib
+
+    mats = sorted(set(int(v) for v in np.unique(el_mat_i)))
+    if mats and mats[0] < 1:
+        raise ValueError("native assembly needs a material for every element")
+    D = np.zeros((len(matls), 36))
+    rho = np.zeros(len(matls))
+    for i, m in enumerate(matls):
+        Dm, r = _material(matls, i + 1)
+        Dm = np.asarray(Dm, dtype=np.float64)
+        D[i, :Dm.size] = Dm.ravel() if Dm.ndim else [float(Dm)]
+        rho[i] = r
+    mask = 0
+    for d in (fe_options["dof_to_use"] or []):
+        mask |= 1 << (int(d) - 1)
+    typ = np.asarray([_NATIVE_TYPES[t] for t in el_typ_i], dtype=np.int32)
+    nn = max(3, els.shape[1])
+    els0 = np.full((els.shape[0], nn), -1, dtype=np.int32)
+    els0[:, :els.shape[1]] = np.where(els > 0, els - 1, -1)
+    rowptr, col, kval, m, c, gl = _lib.assemble_native(
+        nds, els0, typ, el_mat_i - 1, el_abs_i, D, rho, mask, fe_options["damping_power_law"],
+        fe_options["max_damping"], fe_options["max_stiffness_reduction"])
+    n = m.size
+    K = sp.csr_matrix((kval, col, rowptr), shape=(n, n))
+    K.has_sorted_indices = True
+    M = sp.diags(m, format="csr")
+    C = sp.diags(c, format="csr")
+    C.eliminate_zeros()
+    return K, C, M, gl
 
 
 def _reduce(A, valid):

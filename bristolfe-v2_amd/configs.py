@@ -139,3 +139,90 @@ def array_plate(width=20e-3, height=10e-3, el_size=0.1e-3, n_el=64, pitch=0.3e-3
     return dict(mod=mod, matls=matls, fe_options=opts, K=K, C=C, M=M, gl_lookup=gl, time=t, dt=dt,
                 el_gi=gi, frcs=sig[None, :], nodes=nodes,
                 name=f"array_plate {n_el} el pitch {pitch*1e3:g} mm")
+
+
+def adjoint_plate(width=30e-3, height=15e-3, el_size=0.05e-3, n_el=16, pitch=1e-3, centre_freq=3e6,
+                  cycles=5, n_t=5000, ckpt_every=250, incl_radius=2.0e-3, seed=77):
+    """Steel plate with a 16-element surface array for gradient-based
+    inversion: the parametrised model, the shots (each array element one
+    transmitter, DOF 2; all elements receive), and the target parameters of
+    a seeded smooth-blob inclusion (stiffness x 0.7, density x 0.9)."""
+    from .adjoint import ParametrizedModel, Shots
+
+    rng = np.random.default_rng(seed)
+    matls = [steel()]
+    mod = _typ(_m.fn_rectangular_structured_mesh(_rect(0, 0, width, height), el_size), matls)
+    pm = ParametrizedModel(mod, matls, {"dof_to_use": [1, 2]})
+    dt = _m.stable_time_step(pm.K0, pm.M0.diagonal(), iters=3000, tol=1e-12)
+    t = np.arange(n_t) * dt
+    sig = _m.hann_toneburst(t, centre_freq, cycles)
+    xs = width / 2 + (np.arange(n_el) - (n_el - 1) / 2) * pitch
+    nodes = _m.find_nodes_at_points(mod["nds"], np.stack([xs, np.full(xs.size, height)], axis=1))
+    dofs = pm.gl[nodes - 1, 1] - 1
+    shots = Shots([np.array([d]) for d in dofs], sig, dofs, n_t, dt, ckpt_every)
+    blob = _m.fn_create_smooth_random_blob(0.5, 3, 64, rng) * incl_radius
+    centre = np.array([width * (0.4 + 0.2 * rng.random()), height * (0.35 + 0.2 * rng.random())])
+    ctr = _m.fn_calc_element_centres(mod["nds"], mod["els"])[pm.el_index]
+    inside = _m.inpolygon(ctr[:, 0], ctr[:, 1], blob[:, 0] + centre[0], blob[:, 1] + centre[1])
+    s_true = np.where(inside, 0.7, 1.0)
+    rho_true = np.where(inside, 0.9, 1.0) * pm.rho0
+    return dict(model=pm, shots=shots, s_true=s_true, rho_true=rho_true, inside=inside, dt=dt,
+                name=(f"adjoint_plate steel {width*1e3:g}x{height*1e3:g} mm h={el_size*1e3:g} mm, "
+                      f"{n_el} tx/rx, {n_t} steps"))
+
+
+def subdomain_block(n_dof_target, el_size=0.1e-3, rim=1e-3, n_src=64, rng=None):
+    """Steel rectangle (2:1) of about n_dof_target DOF (DOF 1, 2) with an
+    absorbing rim on every side, and n_src forcing/monitoring nodes spread
+    along the inner top edge."""
+    rng = np.random.default_rng(rng)
+    nodes = n_dof_target / 2.0
+    H = np.sqrt(nodes / 2.0) * el_size
+    W = 2.0 * H
+    matls = [steel()]
+    mod = _typ(_m.fn_rectangular_structured_mesh(_rect(0, 0, W, H), el_size), matls)
+    inner = _rect(rim, rim, W - rim, H - rim)
+    mod = _m.fn_add_absorbing_layer(mod, inner, rim)
+    opts = {"dof_to_use": [1, 2]}
+    K, C, M, gl = _assemble(mod, matls, opts)
+    xs = np.sort(rim + (W - 2 * rim) * rng.random(n_src))
+    nd = _m.find_nodes_at_points(mod["nds"], np.stack([xs, np.full(n_src, H - rim)], axis=1))
+    return dict(K=K, C=C, M=M, gl=gl, src=gl[nd - 1, 1] - 1, n=K.shape[0])
+
+
+def fmc_multimodel(n_el=64, pitch=0.3e-3, n_sub=8, sub_dof=(1e5, 1e6), n_t=3000, seed=4,
+                   centre_freq=2e6, cycles=5, with_subdomains=True):
+    """Full-matrix capture on the 20 mm x 10 mm plate (64 elements at 0.3 mm
+    pitch, one source column per transmitting element, every element
+    receiving) plus n_sub subdomain models of seeded uneven size, each with 64
+    source columns on its inner boundary, all marched as ONE block-diagonal
+    system with 64 columns. Returns the combined K, C, M and 0-based forcing
+    entries (dof, column) and monitored DOFs."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    main = array_plate(n_el=n_el, pitch=pitch, n_t=n_t, centre_freq=centre_freq, cycles=cycles)
+    blocks = [dict(K=main["K"], C=main["C"], M=main["M"], src=main["el_gi"] - 1, n=main["K"].shape[0])]
+    if with_subdomains:
+        sizes = rng.uniform(sub_dof[0], sub_dof[1], n_sub)
+        for k in range(n_sub):
+            blocks.append(subdomain_block(sizes[k], n_src=n_el, rng=rng))
+    offs = np.cumsum([0] + [b["n"] for b in blocks])
+    K = sp.block_diag([b["K"] for b in blocks], format="csr")
+    C = sp.block_diag([b["C"] for b in blocks], format="csr")
+    M = sp.block_diag([b["M"] for b in blocks], format="csr")
+    # same material and element size everywhere: the plate's stable step holds
+    # for the subdomains too (their absorbing rims only lower the stiffness)
+    dt = main["dt"]
+    fdof, fsrc, mon = [], [], []
+    for b, o in zip(blocks, offs[:-1]):
+        fdof.append(b["src"] + o)
+        fsrc.append(np.arange(n_el))
+        mon.append(b["src"] + o)
+    t = np.arange(n_t) * dt
+    return dict(K=K, C=C, M=M, dt=dt, time=t, frcs=_m.hann_toneburst(t, centre_freq, cycles)[None, :],
+                frc_dof=np.concatenate(fdof), frc_src=np.concatenate(fsrc), hist_dof=np.concatenate(mon),
+                n_src=n_el, model_sizes=[int(b["n"]) for b in blocks], offsets=offs,
+                main_el_dof=main["el_gi"] - 1,
+                name=(f"fmc {n_el} tx on plate_2d 20x10 mm h=0.1 mm (pitch {pitch*1e3:g} mm)"
+                      + (f" + {n_sub} subdomain models" if with_subdomains else "")))

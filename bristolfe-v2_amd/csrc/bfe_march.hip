@@ -34,7 +34,6 @@ namespace {
 
 constexpr int kBlock = 256;              // threads per workgroup
 constexpr int kRowsSingle = kBlock;      // rows per workgroup, single source
-constexpr int kRowsMulti = kBlock / 64;  // rows per workgroup, multi source
 
 struct hip_failure : std::runtime_error {
   using std::runtime_error::runtime_error;
@@ -184,23 +183,57 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // ---------------------------------------------------------------------------
 // Multi right-hand side step (S <= 64 sources, u stored row-major n x S).
 // One wavefront per row, lane = source: the row's K entries are wave-uniform
-// and the u1 gathers read S contiguous values.
+// and the u1 gathers read S contiguous values. A workgroup holds blockDim/64
+// rows (tunable); the per-row lists (forcing, monitors, interface) are indexed
+// by row. remap = 1 applies the XCD-aware workgroup order.
 template <typename T, bool DAMPED>
-__global__ __launch_bounds__(kBlock) void k_step_multi(
-    int64_t n, int32_t S, int64_t nblk, const int64_t* __restrict__ off,
+__global__ __launch_bounds__(512) void k_step_multi(
+    int64_t n, int32_t S, int64_t nblk, int32_t remap, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
     BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
     int32_t t, int32_t q, T* __restrict__ hist_prev, IfaceRows<T> I) {
-  const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t row = lb * kRowsMulti + (threadIdx.x >> 6);
+  const int64_t lb = remap ? xcd_block(blockIdx.x, nblk) : static_cast<int64_t>(blockIdx.x);
+  const int64_t row = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= n || lane >= S) return;
+  if (row >= n) return;  // row is uniform across the wavefront
+  const bool active = lane < S;
   const int64_t sl = row >> 6;
-  const int64_t e = off[sl + 1];
+  const int64_t b0 = off[sl] + (row & 63);
+  const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries of this row
+  // The row's entries are loaded once, one per lane, and broadcast with
+  // cross-lane shuffles: the gathers of u1 then issue back to back.
   T acc = T(0);
-  for (int64_t p = off[sl] + (row & 63); p < e; p += 64)
-    acc += val[p] * u1[static_cast<int64_t>(col[p]) * S + lane];
+  for (int k0 = 0; k0 < w; k0 += 64) {
+    const int cnt = min(64, w - k0);
+    T myv = T(0);
+    int myc = 0;
+    if (lane < cnt) {
+      myv = val[b0 + 64 * static_cast<int64_t>(k0 + lane)];
+      myc = col[b0 + 64 * static_cast<int64_t>(k0 + lane)];
+    }
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+      const int c0 = __shfl(myc, k), c1 = __shfl(myc, k + 1);
+      const int c2 = __shfl(myc, k + 2), c3 = __shfl(myc, k + 3);
+      const T v0 = __shfl(myv, k), v1 = __shfl(myv, k + 1);
+      const T v2 = __shfl(myv, k + 2), v3 = __shfl(myv, k + 3);
+      T x0 = T(0), x1 = T(0), x2 = T(0), x3 = T(0);
+      if (active) {
+        x0 = u1[static_cast<int64_t>(c0) * S + lane];
+        x1 = u1[static_cast<int64_t>(c1) * S + lane];
+        x2 = u1[static_cast<int64_t>(c2) * S + lane];
+        x3 = u1[static_cast<int64_t>(c3) * S + lane];
+      }
+      acc += v0 * x0 + v1 * x1 + v2 * x2 + v3 * x3;
+    }
+    for (; k < cnt; ++k) {
+      const int c0 = __shfl(myc, k);
+      const T v0 = __shfl(myv, k);
+      if (active) acc += v0 * u1[static_cast<int64_t>(c0) * S + lane];
+    }
+  }
+  if (!active) return;
   const int64_t me = row * S + lane;
   const T a = u1[me], b = u2[me];
   T un;
@@ -210,25 +243,23 @@ __global__ __launch_bounds__(kBlock) void k_step_multi(
   } else {
     un = (T(2) * a - b) - acc;
   }
-  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i)
-    if (L.f_row[i] == row && L.f_src[i] == lane)
-      un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
-  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i)
-    if (L.i_row[i] == row) {
-      const int32_t k = L.i_k[i];
-      T c1 = T(0), c2 = T(0);
-      for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
-        const int64_t c = static_cast<int64_t>(I.ccol[p]) * S + lane;
-        c1 += I.cval[p] * u1[c];
-        c2 += I.cval[p] * u2[c];
-      }
-      un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
-      if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un;
+  for (int32_t i = L.fptr[row], ie = L.fptr[row + 1]; i < ie; ++i)
+    if (L.f_src[i] == lane) un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+  for (int32_t i = L.iptr[row], ie = L.iptr[row + 1]; i < ie; ++i) {
+    const int32_t k = L.i_k[i];
+    T c1 = T(0), c2 = T(0);
+    for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
+      const int64_t c = static_cast<int64_t>(I.ccol[p]) * S + lane;
+      c1 += I.cval[p] * u1[c];
+      c2 += I.cval[p] * u2[c];
     }
+    un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
+    if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un;
+  }
   u[me] = un;
   if (hist_prev != nullptr)
-    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i)
-      if (L.m_row[i] == row) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a;
+    for (int32_t i = L.mptr[row], ie = L.mptr[row + 1]; i < ie; ++i)
+      hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a;
 }
 
 // ---------------------------------------------------------------------------
@@ -440,6 +471,8 @@ struct bfe_plan_s {
   DevBuf is_ptr, is_row, is_k, im_ptr, im_row, im_k;
   bfe_run_stats stats{};
   int32_t timing_every = 0;
+  int32_t multi_rows = 4;   // rows (wavefronts) per workgroup of k_step_multi
+  int32_t xcd_remap = 1;    // XCD-aware workgroup order
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
   // adjoint support (defined further down)
   std::map<int32_t, std::shared_ptr<void>> adj_records;  // AdjointRecord per slot
@@ -541,11 +574,11 @@ void upload_plan(bfe_plan_s& H, const bfe::Plan& P) {
   if (P.damped) H.beta = upload(cast_vec<T>(P.beta));
   const int64_t n = P.n;
   const int64_t nb_s = (n + kRowsSingle - 1) / kRowsSingle;
-  const int64_t nb_m = (n + kRowsMulti - 1) / kRowsMulti;
+  const int64_t nb_m = n;  // multi-source lists are per row
   std::vector<int32_t> ks(P.iface.size());
   for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
   Grouped gs = group_by_block(nb_s, kRowsSingle, P.iface, &ks, nullptr);
-  Grouped gm = group_by_block(nb_m, kRowsMulti, P.iface, &ks, nullptr);
+  Grouped gm = group_by_block(nb_m, 1, P.iface, &ks, nullptr);
   H.is_ptr = upload(gs.ptr); H.is_row = upload(gs.row); H.is_k = upload(gs.a);
   H.im_ptr = upload(gm.ptr); H.im_row = upload(gm.row); H.im_k = upload(gm.a);
   if (!P.iface.empty()) {
@@ -1037,10 +1070,10 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
-  const int64_t nblk = (n + kRowsMulti - 1) / kRowsMulti;
+  const int32_t rpb = H.multi_rows;
+  const int64_t nblk = (n + rpb - 1) / rpb;
   RunBuffers R;
-  make_run_buffers<T>(R, H, kRowsMulti, nblk, S, frc_idx, frc_src, p, frcs, frc_rows, n_t,
-                      hist_idx, q);
+  make_run_buffers<T>(R, H, 1, n, S, frc_idx, frc_src, p, frcs, frc_rows, n_t, hist_idx, q);
   const BlockLists L = R.lists(H, true);
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * S * n_t);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * S * 3);
@@ -1064,12 +1097,12 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
     if (H.damped)
-      hipLaunchKernelGGL((k_step_multi<T, true>), dim3(nblk), dim3(kBlock), 0, st, n, S, nblk,
-                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+      hipLaunchKernelGGL((k_step_multi<T, true>), dim3(nblk), dim3(64 * rpb), 0, st, n, S, nblk,
+                         H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
                          H.beta.as<T>(), u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
     else
-      hipLaunchKernelGGL((k_step_multi<T, false>), dim3(nblk), dim3(kBlock), 0, st, n, S, nblk,
-                         H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+      hipLaunchKernelGGL((k_step_multi<T, false>), dim3(nblk), dim3(64 * rpb), 0, st, n, S, nblk,
+                         H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
                          static_cast<const T*>(nullptr), u1, u2, u, L, falpha, F, frc_rows, ti,
                          q, hp, IR);
     sampler.after(t);
@@ -1249,6 +1282,24 @@ bfe_status bfe_run_batched(bfe_handle h, int32_t n_t, int32_t n_src, const int32
     else
       run_batched<float>(*h, n_t, n_src, frc_idx, frc_src, p, frcs, frc_rows, hist_idx, q,
                          hist_out, ti_start_out);
+  });
+}
+
+bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    switch (key) {
+      case BFE_TUNE_MULTI_ROWS_PER_BLOCK:
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+          throw std::invalid_argument("rows per workgroup must be 1, 2, 4 or 8");
+        h->multi_rows = value;
+        break;
+      case BFE_TUNE_XCD_REMAP:
+        h->xcd_remap = value ? 1 : 0;
+        break;
+      default:
+        throw std::invalid_argument("unknown tuning key");
+    }
   });
 }
 

@@ -193,6 +193,16 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
                        double* g_stiffness, double* g_density);
 
 /*
+ * bfe_set_tuning — launch shape of the multi-source march (no counterpart in
+ * the reference): rows per workgroup of the one-wavefront-per-row kernel
+ * (1, 2, 4 or 8; default 4) and the XCD-aware workgroup order (default on).
+ * Results do not depend on either; the autotuner picks them by timing.
+ */
+#define BFE_TUNE_MULTI_ROWS_PER_BLOCK 1
+#define BFE_TUNE_XCD_REMAP 2
+bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value);
+
+/*
  * bfe_set_kernel_timing — sample the duration of the main step kernel.
  * No counterpart in the reference (its only timing is the wall clock of
  * v6:220). every > 0 brackets every every-th step-kernel launch with HIP
@@ -200,6 +210,32 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * bfe_run_stats. every <= 0 switches sampling off (the default).
  */
 bfe_status bfe_set_kernel_timing(bfe_handle h, int32_t every);
+
+/*
+ * Native global assembly for large meshes (host, FP64, multi-threaded).
+ * Replaces code/internal/fn_build_global_matrices_v5.m:50-159 for meshes
+ * of CPE3 (type 0), AC2D3 (1) and C3D4 (2) elements without interface
+ * elements, with the same DOF numbering, absorbing modifications and column
+ * compaction; K is produced row by row, so memory stays O(nnz).
+ *   nds (n_nodes x dim, row-major), els (n_el x nper, 0-based, -1 unused),
+ *   el_type, el_mat (0-based), el_abs (absorbing index or NULL),
+ *   mat_D (n_mat x 36, 6x6 row-major; AC2D3 reads D[0] = bulk modulus),
+ *   mat_rho (n_mat), dof_mask (bit d-1 = DOF d in use; 0 = all).
+ * bfe_assemble_pattern returns the sizes and a work object that
+ * bfe_assemble_fill consumes: CSR of K (rowptr n_dof+1, col/kval nnz),
+ * lumped mass m and damping diagonal c (n_dof), gl_lookup (n_nodes x max_df,
+ * row-major, 1-based, 0 = absent).
+ */
+bfe_status bfe_assemble_pattern(int64_t n_nodes, int32_t dim, const double* nds, int64_t n_el,
+                                int32_t nper, const int32_t* els, const int32_t* el_type,
+                                const int32_t* el_mat, const double* el_abs, int32_t n_mat,
+                                const double* mat_D, const double* mat_rho, int32_t dof_mask,
+                                double damping_power_law, double max_damping,
+                                double max_stiffness_reduction, int64_t* n_dof, int64_t* nnz,
+                                int32_t* max_df, void** work);
+bfe_status bfe_assemble_fill(void* work, int64_t* rowptr, int32_t* col, double* kval, double* m,
+                             double* c, int64_t* gl_lookup);
+const char* bfe_assemble_last_error(void);
 
 bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
 bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);

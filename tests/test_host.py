@@ -152,3 +152,39 @@ def test_stable_time_step_matches_dense_eigenvalue():
     assert np.isclose(dt, 2 / np.sqrt(lam), rtol=1e-3)
     vmax, _ = PKG.mesh.fn_estimate_max_min_vels(matls)
     assert np.isclose(PKG.mesh.fn_get_suitable_time_step(matls, 1e-4), 1e-4 / vmax / np.sqrt(2))
+
+
+@pytest.mark.parametrize("case", ["solid", "fluid_solid", "tets"])
+@pytest.mark.parametrize("dofs", [[], [1, 2]])
+def test_native_assembly_matches_numpy_and_oracle(case, dofs):
+    m = PKG.mesh
+    if case == "solid":
+        mod, matls = solid_plate(2e-3, 1e-3, 0.125e-3)
+        inner = np.array([[0.5e-3, 0.25e-3], [1.5e-3, 0.25e-3], [1.5e-3, 5e-3], [0.5e-3, 5e-3]])
+        mod = m.fn_add_absorbing_layer(mod, inner, 0.25e-3)
+    elif case == "fluid_solid":
+        mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.25e-3)
+        keep = mod["el_typ_i"] != "ASI2D2"   # native path: no interface elements
+        for k in ("els", "el_mat_i", "el_abs_i", "el_typ_i"):
+            mod[k] = np.asarray(mod[k])[keep]
+        if dofs:
+            dofs = [1, 2, 4]
+    else:
+        mod = m.cubic_structured_tet_mesh(1e-3, 0.25e-3)
+        mod["el_typ_i"] = np.array(["C3D4"] * len(mod["el_mat_i"]), dtype=object)
+        mod["el_abs_i"] = np.linspace(0, 1, len(mod["el_mat_i"])) ** 2
+        matls = [steel()]
+    opts = {"dof_to_use": dofs, "damping_power_law": 2}
+    K1, C1, M1, gl1 = assemble(mod, matls, dict(opts, assembly_backend="numpy"))
+    K2, C2, M2, gl2 = assemble(mod, matls, dict(opts, assembly_backend="native"))
+    np.testing.assert_array_equal(gl1, gl2)
+    for A, B in ((K1, K2), (C1, C2), (M1, M2)):
+        assert A.shape == B.shape
+        assert abs(A - B).max() <= 1e-13 * max(abs(A).max(), 1e-300)
+    # stored patterns may differ only where element contributions cancel to
+    # rounding level in one summation order and exactly in the other
+    if case != "tets" or not dofs:
+        K3, C3, M3, gl3 = oa.build_global_matrices(mod["nds"], mod["els"], mod["el_mat_i"],
+                                                   mod["el_abs_i"], list(mod["el_typ_i"]), matls, opts)
+        np.testing.assert_array_equal(gl2, gl3)
+        assert abs(K2 - K3).max() <= 1e-13 * abs(K3).max()
