@@ -47,10 +47,12 @@ def build_workload(name, n_t=None):
         return configs.coupled_2d(**kw)
     if name == "plate_2d":
         return configs.plate_2d(**kw)
-    m = re.fullmatch(r"coupled_2d@([0-9.]+)", name)
-    if m:  # reduced element size, for quick runs
-        h = float(m.group(1)) * 1e-3
-        return configs.coupled_2d(el_size=h, **kw)
+    if name == "cube_3d":
+        return configs.cube_3d(**kw)
+    m = re.fullmatch(r"(coupled_2d|cube_3d)@([0-9.]+)", name)
+    if m:  # other element size (mm), for quick runs
+        h = float(m.group(2)) * 1e-3
+        return getattr(configs, m.group(1))(el_size=h, **kw)
     raise SystemExit(f"unknown workload {name}")
 
 

@@ -87,13 +87,13 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
         raise ValueError("Length of absorbing index vector must equal number of elements")
 
     dof_to_use = fe_options["dof_to_use"]
-    unique_typs = sorted(set(el_typ_i.tolist()))
+    unique_typs, typ_code = _type_codes(el_typ_i)
     backend = fe_options["assembly_backend"]
     native_ok = set(unique_typs) <= set(_NATIVE_TYPES) and not fe_options["interface_damping_factor"]
     if backend == "native" and not native_ok:
         raise ValueError("native assembly supports CPE3, AC2D3 and C3D4 elements only")
     if backend == "native" or (backend == "auto" and native_ok and els.shape[0] > 2_000_000):
-        return _assemble_native(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_options)
+        return _assemble_native(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls, fe_options)
     unique_df, _ = find_dof_in_use_and_max_dof_per_el(unique_typs, dof_to_use)
     max_df = int(unique_df.max())
     no_nds = nds.shape[0]
@@ -105,9 +105,9 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
     m_diag = np.zeros(total_dof)
     c_diag = np.zeros(total_dof)
     Ki, Kj, Kv, Ci, Cj, Cv = [], [], [], [], [], []
-    for typ in unique_typs:
+    for ti, typ in enumerate(unique_typs):
         fn = element_function(typ)
-        el_i = el_typ_i == typ
+        el_i = typ_code == ti
         for mi in np.unique(el_mat_i[el_i]):
             sel = np.nonzero((el_mat_i == mi) & el_i)[0]
             if sel.size == 0:
@@ -177,11 +177,21 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
     return K, C, M, gl_lookup
 
 
-def _assemble_native(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_options):
+def _type_codes(el_typ_i):
+    """Sorted distinct element types and each element's index into them. A
+    mesh of one element type (the common case on large meshes) is detected
+    with one vectorised comparison instead of a Python pass per element."""
+    if el_typ_i.size and bool(np.all(el_typ_i == el_typ_i[0])):
+        return [el_typ_i[0]], np.zeros(el_typ_i.size, dtype=np.int32)
+    uniq = sorted(set(el_typ_i.tolist()))
+    lut = {t: i for i, t in enumerate(uniq)}
+    return uniq, np.fromiter((lut[t] for t in el_typ_i), dtype=np.int32, count=el_typ_i.size)
+
+
+def _assemble_native(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls, fe_options):
     from . import _lib
 
-    mats = sorted(set(int(v) for v in np.unique(el_mat_i)))
-    if mats and mats[0] < 1:
+    if el_mat_i.size and el_mat_i.min() < 1:
         raise ValueError("native assembly needs a material for every element")
     D = np.zeros((len(matls), 36))
     rho = np.zeros(len(matls))
@@ -193,10 +203,12 @@ def _assemble_native(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_options):
     mask = 0
     for d in (fe_options["dof_to_use"] or []):
         mask |= 1 << (int(d) - 1)
-    typ = np.asarray([_NATIVE_TYPES[t] for t in el_typ_i], dtype=np.int32)
+    typ = np.asarray([_NATIVE_TYPES[t] for t in unique_typs], dtype=np.int32)[typ_code]
     nn = max(3, els.shape[1])
     els0 = np.full((els.shape[0], nn), -1, dtype=np.int32)
-    els0[:, :els.shape[1]] = np.where(els > 0, els - 1, -1)
+    for s0 in range(0, els.shape[0], _CHUNK):
+        e = els[s0:s0 + _CHUNK]
+        els0[s0:s0 + e.shape[0], :els.shape[1]] = np.where(e > 0, e - 1, -1)
     rowptr, col, kval, m, c, gl = _lib.assemble_native(
         nds, els0, typ, el_mat_i - 1, el_abs_i, D, rho, mask, fe_options["damping_power_law"],
         fe_options["max_damping"], fe_options["max_stiffness_reduction"])

@@ -14,6 +14,11 @@ forcing (1-based global indices + signals) and the monitored DOFs.
                 0.025 mm: 7,690,403 DOF before the void).
   array_plate   steel plate with a linear array of surface sources (one
                 source per element, full-matrix capture).
+  cube_3d       steel cube of linear tetrahedra (C3D4, 6 per cubic cell),
+                absorbing layers on the four side faces, a toneburst normal
+                force on a 3 x 3-node patch at the top-face centre and 16
+                top-face monitors (20 mm at 0.1 mm: 8,120,601 nodes,
+                48,000,000 tets, 24,361,803 DOF).
 
 Time step: 0.9 * 2 / omega_max, omega_max^2 the largest eigenvalue of
 M^-1 K, estimated on a patch of the same element size and materials (the
@@ -66,6 +71,54 @@ def patch_time_step(matls, el_size, dof_to_use, safety=0.9):
         K, C, M, _ = _assemble(mod, [mat], {"dof_to_use": dof_to_use})
         dts.append(_m.stable_time_step(K, M.diagonal(), safety=safety, iters=2000, tol=1e-12))
     return min(dts)
+
+
+def patch_time_step_tets(mat, el_size, safety=0.9, cells=8):
+    """Stable dt from a cells^3 patch of the structured tet mesh."""
+    mod = _m.cubic_structured_tet_mesh(cells * el_size, el_size)
+    mod["el_typ_i"] = np.array(["C3D4"] * len(mod["el_mat_i"]), dtype=object)
+    K, C, M, _ = _assemble(mod, [mat], {"dof_to_use": []})
+    return _m.stable_time_step(K, M.diagonal(), safety=safety, iters=2000, tol=1e-12)
+
+
+def steel_3d():
+    return dict(steel(), el_typ="C3D4")
+
+
+def cube_3d(size=20e-3, el_size=0.1e-3, layer=2e-3, n_t=2000, centre_freq=2e6, cycles=3,
+            n_mon=16, mon_pitch=1.5e-3, dt=None, assembly_backend="auto"):
+    matls = [steel_3d()]
+    mod = _m.cubic_structured_tet_mesh(size, el_size)
+    n_el = mod["els"].shape[0]
+    mod["el_typ_i"] = np.full(n_el, "C3D4", dtype=object)
+    mod = _m.add_absorbing_layer_box(mod, (layer, layer, -np.inf), (size - layer, size - layer, np.inf),
+                                     layer)
+    opts = {"dof_to_use": [], "damping_power_law": 2, "assembly_backend": assembly_backend}
+    K, C, M, gl = _assemble(mod, matls, opts)
+    if dt is None:
+        dt = patch_time_step_tets(matls[0], el_size)
+    t = np.arange(n_t) * dt
+    sig = _m.hann_toneburst(t, centre_freq, cycles)
+    nds = mod["nds"]
+    # nodes are numbered x fastest, then y, then z: the top face is the last
+    # (nx * ny) nodes
+    nc = int(round(size / el_size))
+    nx = nc + 1
+    top0 = nx * nx * nc
+    ic = nc // 2
+    patch = np.array([top0 + (ic + a) + nx * (ic + b) for b in (-1, 0, 1) for a in (-1, 0, 1)])
+    assert np.allclose(nds[patch, 2], size)
+    frc = gl[patch, 2]
+    k = int(round(np.sqrt(n_mon)))
+    off = (np.arange(k) - (k - 1) / 2) * mon_pitch
+    X, Y = np.meshgrid(size / 2 + off, size / 2 + off)
+    ii = np.rint(X.ravel() / el_size).astype(np.int64)
+    jj = np.rint(Y.ravel() / el_size).astype(np.int64)
+    mon_nodes = top0 + ii + nx * jj
+    mon = gl[mon_nodes, 2]
+    return dict(mod=mod, matls=matls, fe_options=opts, K=K, C=C, M=M, gl_lookup=gl, time=t, dt=dt,
+                frc_gi=frc, frcs=sig[None, :], hist_gi=mon, src_node=patch + 1,
+                name=f"cube_3d steel {size*1e3:g} mm C3D4 h={el_size*1e3:g} mm")
 
 
 def plate_2d(width=20e-3, height=10e-3, el_size=0.1e-3, n_t=3000, centre_freq=2e6, cycles=5,

@@ -111,12 +111,20 @@ def cubic_structured_tet_mesh(size, el_size, origin=(0.0, 0.0, 0.0)):
          base + dz, base + dx + dz, base + dx + dy + dz, base + dy + dz]
     # Kuhn split along v0 -> v6
     tets = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
-    els = np.concatenate([np.stack([v[a], v[b], v[c], v[d]], axis=1) for a, b, c, d in tets])
-    # orient every tet positively
-    X4 = nds[els - 1]
-    J = np.linalg.det(X4[:, 1:, :] - X4[:, :1, :])
-    flip = J < 0
-    els[flip, 1], els[flip, 2] = els[flip, 2].copy(), els[flip, 1].copy()
+    # every cell has the same shape, so one cell fixes each tet type's
+    # orientation: swap two vertices of the negatively oriented types
+    corner = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                       [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], dtype=np.float64)
+    order = []
+    for t in tets:
+        x = corner[list(t)]
+        if np.linalg.det(x[1:] - x[:1]) < 0:
+            t = (t[0], t[2], t[1], t[3])
+        order.append(t)
+    els = np.empty((len(tets) * base.size, 4), dtype=np.int64)
+    for k, t in enumerate(order):
+        for j, a in enumerate(t):
+            els[k * base.size:(k + 1) * base.size, j] = v[a]
     return _model(nds, els)
 
 
@@ -158,10 +166,14 @@ def inpolygon(px, py, vx, vy):
 
 def fn_calc_element_centres(nds, els):
     els = np.asarray(els)
-    valid = els > 0
-    idx = np.where(valid, els - 1, 0)
-    pts = np.asarray(nds)[idx] * valid[:, :, None]
-    return pts.sum(axis=1) / valid.sum(axis=1)[:, None]
+    nds = np.asarray(nds)
+    out = np.empty((els.shape[0], nds.shape[1]))
+    for s0 in range(0, els.shape[0], 1 << 20):  # bounded temporaries on large meshes
+        e = els[s0:s0 + (1 << 20)]
+        valid = e > 0
+        pts = nds[np.where(valid, e - 1, 0)] * valid[:, :, None]
+        out[s0:s0 + e.shape[0]] = pts.sum(axis=1) / valid.sum(axis=1)[:, None]
+    return out
 
 
 def fn_elements_in_region(mod, region):
@@ -222,6 +234,24 @@ def fn_add_absorbing_layer(mod, abs_bdry_pts, abs_bdry_thickness):
     a[inside] = 0.0
     mod = dict(mod)
     mod["el_abs_i"] = a
+    return mod
+
+
+def add_absorbing_layer_box(mod, inner_lo, inner_hi, thickness):
+    """3D form of fn_add_absorbing_layer (code/fn_add_absorbing_layer.m:10-17)
+    for a box-shaped inner region: a_e = clip(d_e / thickness, 0, 1) with d_e
+    the distance from the element centroid to the box [inner_lo, inner_hi]
+    (0 inside). An infinite bound leaves that axis without a layer, e.g.
+    (x0, y0, -inf)..(x1, y1, inf) puts layers on the four side faces only."""
+    c = fn_calc_element_centres(mod["nds"], mod["els"])
+    lo = np.asarray(inner_lo, dtype=np.float64)
+    hi = np.asarray(inner_hi, dtype=np.float64)
+    d2 = np.zeros(c.shape[0])
+    for k in range(c.shape[1]):
+        g = np.maximum(np.maximum(lo[k] - c[:, k], c[:, k] - hi[k]), 0.0)
+        d2 += g * g
+    mod = dict(mod)
+    mod["el_abs_i"] = np.clip(np.sqrt(d2) / thickness, 0.0, 1.0)
     return mod
 
 

@@ -209,3 +209,19 @@ def test_batched_tuning_does_not_change_results():
         assert rel_l2(h, ref) < 1e-13
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_cube_3d_tets_parity(mode):
+    """C3D4 cube with side-face absorbing layers, 3 x 3-node patch source,
+    16 top-face monitors (the 3D workload at 1 mm elements)."""
+    print("case: cube_3d tets", mode, flush=True)
+    from bristolfe_v2_amd import configs
+    c = configs.cube_3d(el_size=1e-3, n_t=400)
+    K, C, M = c["K"], c["C"], c["M"]
+    hi = np.concatenate([c["hist_gi"], np.arange(1, K.shape[0] + 1, 331)])
+    h, _, _, _ = _gpu(K, C, M, c["time"], c["frc_gi"], c["frcs"], hi, mode=mode)
+    r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, c["time"], c["frc_gi"], c["frcs"], hi,
+                                                solver_mode=mode)
+    assert np.linalg.norm(r[:16]) > 0
+    assert rel_l2(h, r) < TOL

@@ -188,3 +188,28 @@ def test_native_assembly_matches_numpy_and_oracle(case, dofs):
                                                    mod["el_abs_i"], list(mod["el_typ_i"]), matls, opts)
         np.testing.assert_array_equal(gl2, gl3)
         assert abs(K2 - K3).max() <= 1e-13 * abs(K3).max()
+
+
+def test_box_absorbing_layer_3d():
+    m = PKG.mesh
+    mod = m.cubic_structured_tet_mesh(4e-3, 0.5e-3)
+    mod = m.add_absorbing_layer_box(mod, (1e-3, 1e-3, -np.inf), (3e-3, 3e-3, np.inf), 1e-3)
+    c = m.fn_calc_element_centres(mod["nds"], mod["els"])
+    a = mod["el_abs_i"]
+    gx = np.maximum(np.maximum(1e-3 - c[:, 0], c[:, 0] - 3e-3), 0)
+    gy = np.maximum(np.maximum(1e-3 - c[:, 1], c[:, 1] - 3e-3), 0)
+    np.testing.assert_allclose(a, np.clip(np.hypot(gx, gy) / 1e-3, 0, 1), rtol=0, atol=1e-15)
+    inside = (gx == 0) & (gy == 0)
+    assert inside.any() and np.all(a[inside] == 0) and a.max() > 0.5
+    # no layer on the top and bottom faces: a does not depend on z
+    top = c[:, 2] > 3.5e-3
+    assert np.all(a[top & inside] == 0)
+
+
+def test_tet_mesh_orientation_and_volume():
+    mod = PKG.mesh.cubic_structured_tet_mesh((2e-3, 1e-3, 1.5e-3), 0.5e-3)
+    X = mod["nds"][mod["els"] - 1]
+    J = np.linalg.det(X[:, 1:, :] - X[:, :1, :])
+    assert np.all(J > 0)
+    np.testing.assert_allclose(J.sum() / 6, 2e-3 * 1e-3 * 1.5e-3, rtol=1e-12)
+    assert mod["els"].shape[0] == 6 * 4 * 2 * 3
