@@ -74,10 +74,10 @@ def algorithmic_bytes_per_dof(nnz, n, damped):
     return 12.0 * nnz / n + 4.0 + 24.0 + 8.0 * k
 
 
-def read_traffic(workload):
-    """HBM bytes per k_step launch from the committed rocprofv3 PMC summary,
-    if one exists for this workload (profiles/k_step_<workload>_pmc.md)."""
-    p = ROOT / "profiles" / f"k_step_{workload.split('@')[0]}_pmc.md"
+def read_traffic(workload, kernel="k_step"):
+    """HBM bytes per step-kernel launch from the committed rocprofv3 PMC
+    summary, if one exists for this workload (profiles/<kernel>_<workload>_pmc.md)."""
+    p = ROOT / "profiles" / f"{kernel}_{workload.split('@')[0]}_pmc.md"
     if not p.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
@@ -282,6 +282,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="coupled_2d")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--operator", choices=("auto", "csr", "ebe"), default="auto",
+                    help="stiffness operator of the march (auto: element-by-element when the "
+                         "mesh has the template form)")
     ap.add_argument("--timing-every", type=int, default=16)
     ap.add_argument("--time-steps", type=int, default=None,
                     help="override the workload's number of time steps (profiling runs)")
@@ -308,15 +311,23 @@ def main():
     c = build_workload(args.workload, args.time_steps)
     log(f"workload {c['name']}: n={c['K'].shape[0]} nnz={c['K'].nnz} n_t={len(c['time'])} "
         f"setup {time.perf_counter() - t_setup:.1f} s")
+    from bristolfe_v2_amd import ebe
     from bristolfe_v2_amd.solver import make_plan
 
     K, C, M = c["K"], c["C"], c["M"]
     n = K.shape[0]
     n_t = len(c["time"])
+    op = None
+    if args.operator != "csr":
+        t0 = time.perf_counter()
+        op = ebe.element_operator(c["mod"], c["matls"], c["fe_options"], c["gl_lookup"], n)
+        log(f"element operator: {'none' if op is None else f'{op.tmpl.shape[0]} templates'} "
+            f"({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
-    plan = make_plan(K, C, M, c["dt"])
+    plan = make_plan(K, C, M, c["dt"], element_op=op, operator=args.operator)
     info = plan.info()
     log(f"plan: {info} ({time.perf_counter() - t0:.1f} s)")
+    use_ebe = info["operator_kind"] == 1
     fi = np.asarray(c["frc_gi"]) - 1
     hi = np.asarray(c["hist_gi"]) - 1
     F = c["frcs"]
@@ -344,10 +355,16 @@ def main():
     wall = time.perf_counter() - wall0
     loop_s = float(np.mean(loops)) / 1e3
     value = n * steps_done / loop_s / 1e9
-    bpd = algorithmic_bytes_per_dof(info["nnz_k"], n, info["damped"])
+    if use_ebe:
+        bpd = ebe.ebe_bytes_per_dof(op, n, info["damped"])
+        kname = (f"k_step_ebe<double,{'true' if info['damped'] else 'false'},{op.el_nodes.shape[1]},"
+                 f"{op.dpn}> (element-by-element gather + central-difference update)")
+    else:
+        bpd = algorithmic_bytes_per_dof(info["nnz_k"], n, info["damped"])
+        kname = "k_step<double,true> (SELL-64 SpMV + central-difference update)"
     kern_avg_s = kern / max(samples, 1) / 1e3
     achieved = bpd * n / kern_avg_s / 1e9 if samples else None
-    traffic = read_traffic(args.workload)
+    traffic = read_traffic(args.workload, "k_step_ebe" if use_ebe else "k_step")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -369,11 +386,29 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": traffic,
-                     "kernel": "k_step<double,true> (SELL-64 SpMV + central-difference update)",
+                     "kernel": kname,
                      "bytes_per_dof_step": round(bpd, 2),
                      "kernel_avg_us": round(kern_avg_s * 1e6, 2), "kernel_samples": int(samples)},
         "loop_gdof_steps_per_s_wall": round(n * steps_done * args.steps / wall / 1e9, 4),
     }
+    if use_ebe:
+        out["config"]["operator"] = (f"element-by-element: {op.n_el} elements, "
+                                     f"{info['ebe_templates']} templates, {info['ebe_slots']} slots")
+        out["roofline"]["stored_bytes_per_dof_step"] = round(
+            ebe.ebe_stored_bytes_per_dof(op, n, info["ebe_slots"], info["damped"]), 2)
+        # the CSR operator of the same plan, one march, for comparison
+        plan.set_tuning(3, 0)
+        plan.set_kernel_timing(args.timing_every)
+        hc, _, _ = plan.run(n_t, fi, F, hi)
+        st = plan.stats()
+        bc = algorithmic_bytes_per_dof(info["nnz_k"], n, info["damped"])
+        kc = st["step_kernel_ms"] / max(st["step_kernel_samples"], 1) / 1e3
+        out["csr_operator"] = {
+            "gdof_steps_per_s": round(n * st["steps"] / st["loop_ms"] / 1e6, 4),
+            "kernel_avg_us": round(kc * 1e6, 2), "bytes_per_dof_step": round(bc, 2),
+            "achieved_GBs": round(bc * n / kc / 1e9, 1),
+            "history_rel_diff_vs_ebe": float(np.linalg.norm(hc - hist) / max(np.linalg.norm(hc), 1e-300))}
+        plan.set_tuning(3, 1)
     if not args.no_cpu:
         log("cpu baseline ...")
         # the sample's check monitors: the forcing DOFs plus the workload's monitors

@@ -19,11 +19,17 @@ PREC_DOUBLE = 0
 PREC_SINGLE = 1
 TUNE_MULTI_ROWS_PER_BLOCK = 1
 TUNE_XCD_REMAP = 2
+TUNE_OPERATOR = 3
+OPERATOR_CSR = 0
+OPERATOR_EBE = 1
+TUNE_EBE_GROUP = 4
+TUNE_EBE_KERNEL = 5
 
 # Every symbol the header declares.
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
-    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_kernel_timing",
+    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
+    "bfe_set_kernel_timing",
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
     "bfe_assemble_fill", "bfe_assemble_last_error",
@@ -35,7 +41,9 @@ class BfeInfo(C.Structure):
         ("n", C.c_int64), ("nnz_k", C.c_int64), ("nnz_sell", C.c_int64),
         ("n_interface", C.c_int64), ("nnz_interface", C.c_int64), ("nnz_xinv", C.c_int64),
         ("damped", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
-        ("device", C.c_int32), ("device_bytes", C.c_int64),
+        ("device", C.c_int32), ("device_bytes", C.c_int64), ("operator_kind", C.c_int32),
+        ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
+        ("ebe_elements", C.c_int64), ("ebe_check", C.c_double),
     ]
 
 
@@ -92,6 +100,10 @@ def load() -> C.CDLL:
     lib.bfe_adjoint.restype = C.c_int
     lib.bfe_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.bfe_set_tuning.restype = C.c_int
+    lib.bfe_set_element_operator.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64,
+                                             C.c_int32, _i32p, _i32p, _f64p, C.c_int32, _f64p,
+                                             C.c_double]
+    lib.bfe_set_element_operator.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -218,6 +230,18 @@ class Plan:
             int(field_every if n_frames else 0), int(field_is_vel),
             fld.ctypes.data_as(_f64p) if fld is not None else None, C.byref(ti)))
         return hist, fld, int(ti.value)
+
+    def set_element_operator(self, op, check_tol=0.0):
+        """Attach an element-by-element operator (an ``ElementOperator``)."""
+        el_nodes = np.ascontiguousarray(op.el_nodes, np.int32)
+        n_el, npe = el_nodes.shape
+        el_tmpl = np.ascontiguousarray(op.el_tmpl, np.int32).ravel()
+        el_scale = np.ascontiguousarray(op.el_scale, np.float64).ravel()
+        tmpl = np.ascontiguousarray(op.tmpl, np.float64)
+        check(self._lib.bfe_set_element_operator(
+            self._h, int(op.n_nodes), int(op.dpn), n_el, npe, ptr(el_nodes, C.c_int32),
+            ptr(el_tmpl, C.c_int32), ptr(el_scale, C.c_double), int(tmpl.shape[0]),
+            ptr(tmpl, C.c_double), float(check_tol)))
 
     def set_gradient_elements(self, el_dofs, el_K, el_m, el_mu):
         """el_dofs (n_el, k) 0-based (-1 unused), el_K (n_el, k, k), el_m (n_el, k), el_mu (n_el)."""

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/bfe.h"
+#include "bfe_ebe.hpp"
 #include "bfe_plan.hpp"
 
 namespace {
@@ -318,6 +319,269 @@ __global__ void k_snap(int64_t n, int32_t S, const T* __restrict__ u_last,
 }
 
 // ---------------------------------------------------------------------------
+// Element-by-element step (single source). The stiffness is held as
+//   K = sum_e s_e P_e' T[tau_e] P_e
+// with a few element templates T (one per element shape and material, local
+// DOF order node-major) and a per-element scale s_e (the absorbing-layer
+// stiffness reduction), and the global DOF of (node v, component c) is
+// v*DPN + c. Thread v owns node v's DPN rows and gathers its row of K u from
+// the elements around it: slot k of every node holds the element in which the
+// node is local vertex l_k of template tau_k (or -1), so the DPN x (NPE*DPN)
+// template rows a slot needs are the same for every lane and are read through
+// the scalar cache. The slot-major adjacency is read coalesced; element nodes
+// and u1 are gathered. The update, forcing, interface, monitor and field
+// handling are those of k_step on the node's rows.
+constexpr int kEbeSlotPad = 8;  // slot count padding: any load group G of 1, 2, 4, 8 divides it
+
+// acc[c] += sum over the G slots k0..k0+G-1 of s_e T_k(c, :) u1(e), given
+// the slots' table entries e0 and element records nd, s.
+template <typename T, int NPE, int DPN, int G>
+__device__ __forceinline__ void ebe_group_apply(int32_t k0, const int32_t* e0, const int4* nd,
+                                                const T* s, const T* __restrict__ srows,
+                                                const T* __restrict__ u1, T* acc) {
+  constexpr int KE = NPE * DPN;
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const T* R = srows + static_cast<int64_t>(k0 + g) * (DPN * KE);
+    const int32_t w[4] = {nd[g].x, nd[g].y, nd[g].z, nd[g].w};
+    T y[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) y[c] = T(0);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b) {
+#pragma unroll
+      for (int d = 0; d < DPN; ++d) {
+        const T x = u1[static_cast<int64_t>(w[b]) * DPN + d];
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) acc[c] += e0[g] < 0 ? T(0) : s[g] * y[c];
+  }
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void ebe_load_table(int64_t n_nodes, int64_t v, int32_t k0,
+                                               const int32_t* __restrict__ adj, int32_t* e0) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) e0[g] = adj[static_cast<int64_t>(k0 + g) * n_nodes + v];
+}
+
+template <typename T, int G>
+__device__ __forceinline__ void ebe_load_records(const int32_t* e0, const int4* __restrict__ el_nodes,
+                                                 const T* __restrict__ el_scale, int4* nd, T* s) {
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const int32_t e = e0[g] < 0 ? 0 : e0[g];  // an empty slot reads element 0, then drops it
+    nd[g] = el_nodes[e];
+    s[g] = el_scale[e];
+  }
+}
+
+// The update of node v's DPN rows from acc = (K u1) on those rows, with the
+// forcing, interface, field and monitor handling of k_step.
+template <typename T, bool DAMPED, int DPN>
+__device__ __forceinline__ void ebe_finish(
+    int64_t v, int64_t lb, const T* acc, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
+    const IfaceRows<T>& I) {
+  const int64_t r0 = v * DPN;
+  T a[DPN], b[DPN], un[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) {
+    const int64_t r = r0 + c;
+    a[c] = u1[r];
+    b[c] = u2[r];
+    const T ka = alpha[r] * acc[c];
+    if constexpr (DAMPED) {
+      const T be = beta[r];
+      un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
+    } else {
+      un[c] = (T(2) * a[c] - b[c]) - ka;
+    }
+  }
+  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.f_row[i];
+    if (rr / DPN == v) {
+      const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+#pragma unroll
+      for (int c = 0; c < DPN; ++c)
+        if (rr == r0 + c) un[c] += f;
+    }
+  }
+  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.i_row[i];
+    if (rr / DPN == v) {
+      const int32_t k = L.i_k[i];
+      T c1 = T(0), c2 = T(0);
+      for (int64_t j = I.cptr[k], je = I.cptr[k + 1]; j < je; ++j) {
+        c1 += I.cval[j] * u1[I.ccol[j]];
+        c2 += I.cval[j] * u2[I.ccol[j]];
+      }
+#pragma unroll
+      for (int c = 0; c < DPN; ++c)
+        if (rr == r0 + c) {
+          un[c] += I.gfix[k] * b[c] + I.sq * c1 + I.sr * c2;
+          if (I.wI != nullptr) I.wI[k] = un[c];
+        }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) {
+    u[r0 + c] = un[c];
+    if (fld_prev != nullptr) fld_prev[r0 + c] = fld_is_vel ? (a[c] - b[c]) / dt : a[c];
+  }
+  if (hist_prev != nullptr)
+    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i) {
+      const int64_t rr = L.m_row[i];
+      if (rr / DPN == v) {
+#pragma unroll
+        for (int c = 0; c < DPN; ++c)
+          if (rr == r0 + c) hist_prev[L.m_k[i]] = a[c];
+      }
+    }
+}
+
+// Slots in groups of G (n_slot is a multiple of G): a group's table entries,
+// then its element records, are requested before any use, so a wave keeps G
+// independent gathers in flight at each level of the table -> element -> u1
+// chain.
+template <typename T, int NPE, int DPN, int G>
+__device__ __forceinline__ void ebe_gather(int64_t n_nodes, int64_t v, int32_t n_slot,
+                                           const int32_t* __restrict__ adj,
+                                           const int4* __restrict__ el_nodes,
+                                           const T* __restrict__ el_scale,
+                                           const T* __restrict__ srows, const T* __restrict__ u1,
+                                           T* acc) {
+  for (int32_t k0 = 0; k0 < n_slot; k0 += G) {
+    int32_t e0[G];
+    int4 nd[G];
+    T s[G];
+    ebe_load_table<T, G>(n_nodes, v, k0, adj, e0);
+    ebe_load_records<T, G>(e0, el_nodes, el_scale, nd, s);
+    ebe_group_apply<T, NPE, DPN, G>(k0, e0, nd, s, srows, u1, acc);
+  }
+}
+
+template <typename T, bool DAMPED, int NPE, int DPN, int G>
+__global__ __launch_bounds__(kBlock) void k_step_ebe(
+    int64_t n_nodes, int64_t nblk, int32_t n_slot, const int32_t* __restrict__ adj,
+    const int4* __restrict__ el_nodes, const T* __restrict__ el_scale,
+    const T* __restrict__ srows, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
+    IfaceRows<T> I) {
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  const int64_t v = lb * kBlock + threadIdx.x;
+  if (v >= n_nodes) return;
+  T acc[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+  ebe_gather<T, NPE, DPN, G>(n_nodes, v, n_slot, adj, el_nodes, el_scale, srows, u1, acc);
+  ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
+                             hist_prev, fld_prev, fld_is_vel, dt, I);
+}
+
+// ---------------------------------------------------------------------------
+// Mesh patterns: the (slot, vertex) -> neighbour-class tables of the layout
+// (slots and vertices in canonical order) of the structured meshes, so that
+// k_step_ebe_pat can keep each neighbour's u1 in registers. Class SELF is the
+// node itself.
+struct PatKuhn24 {  // cubic cells split into 6 tetrahedra around a main diagonal
+  static constexpr int ID = 1, NPE = 4, NS = 24, NC = 15, SELF = 7;
+  static constexpr int8_t cls[NS][NPE] = {
+      {0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7},
+      {1, 3, 7, 8}, {1, 5, 7, 8}, {2, 3, 7, 9}, {2, 6, 7, 9}, {3, 7, 8, 10}, {3, 7, 9, 10},
+      {4, 5, 7, 11}, {4, 6, 7, 11}, {5, 7, 8, 12}, {5, 7, 11, 12}, {6, 7, 9, 13},
+      {6, 7, 11, 13}, {7, 8, 10, 14}, {7, 8, 12, 14}, {7, 9, 10, 14}, {7, 9, 13, 14},
+      {7, 11, 12, 14}, {7, 11, 13, 14}};
+};
+struct PatTriRect {  // squares split into two right triangles
+  static constexpr int ID = 2, NPE = 3, NS = 6, NC = 7, SELF = 3;
+  static constexpr int8_t cls[NS][NPE] = {{0, 1, 3}, {0, 2, 3}, {1, 3, 4},
+                                          {2, 3, 5}, {3, 4, 6}, {3, 5, 6}};
+};
+struct PatTriIso {  // near-equilateral triangles
+  static constexpr int ID = 3, NPE = 3, NS = 6, NC = 7, SELF = 2;
+  static constexpr int8_t cls[NS][NPE] = {{0, 1, 2}, {0, 2, 3}, {1, 2, 4},
+                                          {2, 3, 6}, {2, 4, 5}, {2, 5, 6}};
+};
+
+// Element-by-element step on a mesh pattern: the NC neighbours' u1 values are
+// gathered once (neighbour table nbr, class-major) and kept in registers;
+// every slot then needs only its table entry and element scale, and its
+// template rows (canonical vertex order) come through the scalar cache.
+template <typename T, bool DAMPED, typename PT, int DPN>
+__global__ __launch_bounds__(kBlock) void k_step_ebe_pat(
+    int64_t n_nodes, int64_t nblk, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
+    const T* __restrict__ srows, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
+    IfaceRows<T> I) {
+  constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  const int64_t v = lb * kBlock + threadIdx.x;
+  if (v >= n_nodes) return;
+  T x[NC][DPN];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    int64_t w = v;
+    if (j != PT::SELF) {
+      const int32_t wj = nbr[static_cast<int64_t>(j) * n_nodes + v];
+      w = wj < 0 ? v : wj;  // an absent neighbour is used only by empty slots
+    }
+#pragma unroll
+    for (int d = 0; d < DPN; ++d) x[j][d] = u1[w * DPN + d];
+  }
+  T acc[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const int32_t e0 = adj[static_cast<int64_t>(k) * n_nodes + v];
+    const T s = el_scale[e0 < 0 ? 0 : e0];
+    const T* R = srows + k * (DPN * KE);
+    T y[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) y[c] = T(0);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+      for (int d = 0; d < DPN; ++d)
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) acc[c] += e0 < 0 ? T(0) : s * y[c];
+  }
+  ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
+                             hist_prev, fld_prev, fld_is_vel, dt, I);
+}
+
+// Pattern id of a layout's class table, 0 if none matches.
+int match_pattern(const bfe::EbeLayout& L) {
+  auto same = [&](auto pat) {
+    using P = decltype(pat);
+    if (L.npe != P::NPE || L.n_slot != P::NS || L.n_cls != P::NC || L.self_cls != P::SELF)
+      return false;
+    for (int k = 0; k < P::NS; ++k)
+      for (int b = 0; b < P::NPE; ++b)
+        if (L.cls[k * P::NPE + b] != P::cls[k][b]) return false;
+    return true;
+  };
+  if (L.n_cls == 0) return 0;
+  if (same(PatKuhn24{})) return PatKuhn24::ID;
+  if (same(PatTriRect{})) return PatTriRect::ID;
+  if (same(PatTriIso{})) return PatTriIso::ID;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Per-element gradients over one checkpoint segment of Ls time steps.
 // Thread e keeps K_e (KMAX x KMAX), the lumped mass per unit density and the
 // last two forward states of its DOFs in registers and walks the segment:
@@ -449,6 +713,22 @@ Grouped group_by_block(int64_t nblk, int64_t rows_per_block, const std::vector<i
   return G;
 }
 
+// Element-by-element operator (k_step_ebe) attached to a handle.
+struct EbeOp {
+  int64_t n_nodes = 0, n_el = 0;
+  int32_t dpn = 0, npe = 0, n_slot = 0, n_tmpl = 0, n_cls = 0;
+  int32_t pattern = 0;  // mesh pattern of k_step_ebe_pat, 0: none
+  DevBuf adj, nodes, scale, srows, alpha, nbr, srows_c;
+  DevBuf iptr, irow, ik;  // interface rows grouped per k_step_ebe workgroup
+  double check_err = 0.0;  // largest relative difference to the CSR step at set-up
+  int64_t bytes() const {
+    int64_t s = 0;
+    for (const DevBuf* b : {&adj, &nodes, &scale, &srows, &alpha, &nbr, &srows_c, &iptr, &irow, &ik})
+      s += static_cast<int64_t>(b->bytes);
+    return s;
+  }
+};
+
 }  // namespace
 
 struct bfe_plan_s {
@@ -477,6 +757,11 @@ struct bfe_plan_s {
   // adjoint support (defined further down)
   std::map<int32_t, std::shared_ptr<void>> adj_records;  // AdjointRecord per slot
   std::shared_ptr<void> grad_elems;  // GradElements
+  std::vector<int32_t> iface_host;   // interface rows (global), for regrouping
+  std::unique_ptr<EbeOp> ebe;        // optional element-by-element operator
+  int32_t use_ebe = 1;               // single-source march uses ebe when set
+  int32_t ebe_group = 8;             // slots per load group of k_step_ebe (2, 4 or 8)
+  int32_t ebe_kernel = 1;            // 1: pattern kernel when the mesh matches one
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
@@ -484,6 +769,7 @@ struct bfe_plan_s {
     }
     adj_records.clear();
     grad_elems.reset();
+    ebe.reset();
     for (DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr, &xcol,
                       &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
       b->release();
@@ -497,6 +783,7 @@ struct bfe_plan_s {
     for (const DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr,
                             &xcol, &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
       s += static_cast<int64_t>(b->bytes);
+    if (ebe) s += ebe->bytes();
     return s;
   }
 };
@@ -790,6 +1077,149 @@ void launch_step(const bfe_plan_s& H, int S, int64_t nblk, const T* u1, const T*
   }
 }
 
+template <typename T, bool D, int NPE, int DPN, int G>
+void launch_ebe_group(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
+                      T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
+                      int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
+                      const IfaceRows<T>& IR) {
+  hipLaunchKernelGGL((k_step_ebe<T, D, NPE, DPN, G>), dim3(nblk), dim3(kBlock), 0, H.stream,
+                     E.n_nodes, nblk, E.n_slot, E.adj.as<int32_t>(), E.nodes.as<int4>(),
+                     E.scale.as<T>(), E.srows.as<T>(), E.alpha.as<T>(),
+                     H.damped ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q,
+                     hp, fp, fld_is_vel, dt, IR);
+}
+
+template <typename T, bool D, int NPE, int DPN>
+void launch_ebe_kernel(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
+                       T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
+                       int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
+                       const IfaceRows<T>& IR) {
+  switch (H.ebe_group) {
+    case 2: launch_ebe_group<T, D, NPE, DPN, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case 8: launch_ebe_group<T, D, NPE, DPN, 8>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    default: launch_ebe_group<T, D, NPE, DPN, 4>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+  }
+}
+
+template <typename T, bool D>
+void launch_ebe_damped(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1,
+                       const T* u2, T* u, const BlockLists& L, const T* falpha, const T* F,
+                       int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel,
+                       T dt, const IfaceRows<T>& IR) {
+  const int key = E.npe * 10 + E.dpn;
+  switch (key) {
+    case 31: launch_ebe_kernel<T, D, 3, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case 32: launch_ebe_kernel<T, D, 3, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case 33: launch_ebe_kernel<T, D, 3, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case 43: launch_ebe_kernel<T, D, 4, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    default: throw std::invalid_argument("unsupported element operator shape");
+  }
+}
+
+template <typename T, bool D, typename PT, int DPN>
+void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
+                    T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
+                    int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
+                    const IfaceRows<T>& IR) {
+  hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN>), dim3(nblk), dim3(kBlock), 0, H.stream,
+                     E.n_nodes, nblk, E.nbr.as<int32_t>(), E.adj.as<int32_t>(), E.scale.as<T>(),
+                     E.srows_c.as<T>(), E.alpha.as<T>(), H.damped ? H.beta.as<T>() : nullptr, u1,
+                     u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR);
+}
+
+template <typename T, bool D>
+void launch_ebe_pattern(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1,
+                        const T* u2, T* u, const BlockLists& L, const T* falpha, const T* F,
+                        int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel,
+                        T dt, const IfaceRows<T>& IR) {
+  const int key = E.pattern * 10 + E.dpn;
+  switch (key) {
+    case PatKuhn24::ID * 10 + 3: launch_ebe_pat<T, D, PatKuhn24, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriRect::ID * 10 + 1: launch_ebe_pat<T, D, PatTriRect, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriRect::ID * 10 + 2: launch_ebe_pat<T, D, PatTriRect, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriRect::ID * 10 + 3: launch_ebe_pat<T, D, PatTriRect, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriIso::ID * 10 + 1: launch_ebe_pat<T, D, PatTriIso, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriIso::ID * 10 + 2: launch_ebe_pat<T, D, PatTriIso, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case PatTriIso::ID * 10 + 3: launch_ebe_pat<T, D, PatTriIso, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    default: throw std::invalid_argument("no pattern kernel for this element operator");
+  }
+}
+
+// Whether a pattern kernel exists for the operator's (pattern, dof per node).
+bool has_pattern_kernel(const EbeOp& E) {
+  if (E.pattern == PatKuhn24::ID) return E.dpn == 3;
+  return (E.pattern == PatTriRect::ID || E.pattern == PatTriIso::ID) && E.dpn >= 1 && E.dpn <= 3;
+}
+
+template <typename T>
+void launch_step_ebe(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
+                     T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
+                     int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
+                     const IfaceRows<T>& IR, bool generic = false) {
+  if (!generic && H.ebe_kernel && E.pattern != 0) {
+    if (H.damped)
+      launch_ebe_pattern<T, true>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
+                                  fld_is_vel, dt, IR);
+    else
+      launch_ebe_pattern<T, false>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
+                                   fld_is_vel, dt, IR);
+    return;
+  }
+  if (H.damped)
+    launch_ebe_damped<T, true>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
+                               fld_is_vel, dt, IR);
+  else
+    launch_ebe_damped<T, false>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
+                                fld_is_vel, dt, IR);
+}
+
+// One step of both operators from the same random state (u2 = 0, no
+// forcing, monitors or interface correction): returns
+// max |u_ebe - u_csr| / max |u_csr|.
+template <typename T>
+double compare_ebe_with_csr(const bfe_plan_s& H, const EbeOp& E, bool generic) {
+  const int64_t n = H.n;
+  const int64_t nb_c = (n + kRowsSingle - 1) / kRowsSingle;
+  const int64_t nb_e = (E.n_nodes + kBlock - 1) / kBlock;
+  std::vector<int32_t> zeros(std::max(nb_c, nb_e) + 1, 0);
+  DevBuf z = upload(zeros);
+  BlockLists L0;
+  L0.fptr = L0.f_row = L0.f_src = L0.f_frow = z.as<int32_t>();
+  L0.mptr = L0.m_row = L0.m_k = z.as<int32_t>();
+  L0.iptr = L0.i_row = L0.i_k = z.as<int32_t>();
+  std::vector<T> x(n);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (int64_t i = 0; i < n; ++i) {
+    st = st * 6364136223846793005ull + 1442695040888963407ull;
+    x[i] = static_cast<T>(static_cast<double>(st >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+  }
+  DevBuf dx = upload(x);
+  DevBuf buf(sizeof(T) * static_cast<size_t>(n) * 3);
+  BFE_HIP(hipMemsetAsync(buf.p, 0, buf.bytes, H.stream));
+  T* u2 = buf.as<T>();
+  T* y1 = u2 + n;
+  T* y2 = u2 + 2 * n;
+  IfaceRows<T> I{};
+  I.wI = nullptr;
+  const T dt = static_cast<T>(H.dt);
+  launch_step<T>(H, 1, nb_c, dx.as<T>(), u2, y1, L0, nullptr, nullptr, 1, 0, 0, nullptr, nullptr,
+                 0, dt, I, nullptr, nullptr);
+  launch_step_ebe<T>(H, E, nb_e, dx.as<T>(), u2, y2, L0, nullptr, nullptr, 1, 0, 0, nullptr,
+                     nullptr, 0, dt, I, generic);
+  BFE_HIP(hipGetLastError());
+  BFE_HIP(hipStreamSynchronize(H.stream));
+  std::vector<T> a(n), b(n);
+  BFE_HIP(hipMemcpy(a.data(), y1, sizeof(T) * n, hipMemcpyDeviceToHost));
+  BFE_HIP(hipMemcpy(b.data(), y2, sizeof(T) * n, hipMemcpyDeviceToHost));
+  double dmax = 0.0, amax = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double d = std::fabs(static_cast<double>(a[i]) - static_cast<double>(b[i]));
+    if (!(d <= dmax)) dmax = d;  // NaN propagates
+    amax = std::max(amax, std::fabs(static_cast<double>(a[i])));
+  }
+  return amax > 0.0 ? dmax / amax : dmax;
+}
+
 template <typename T>
 void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                 const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
@@ -821,11 +1251,20 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
-  const int64_t nblk = (n + kRowsSingle - 1) / kRowsSingle;
+  // the checkpointed forward stays on the CSR operator that the adjoint's
+  // recomputation uses
+  const EbeOp* E = (H.ebe && H.use_ebe && rec == nullptr) ? H.ebe.get() : nullptr;
+  const int64_t nblk = E ? (E->n_nodes + kBlock - 1) / kBlock : (n + kRowsSingle - 1) / kRowsSingle;
+  const int64_t rows_per_block = E ? static_cast<int64_t>(kBlock) * E->dpn : kRowsSingle;
   RunBuffers R;
-  make_run_buffers<T>(R, H, kRowsSingle, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
+  make_run_buffers<T>(R, H, rows_per_block, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
                       hist_idx, q);
-  const BlockLists L = R.lists(H, false);
+  BlockLists L = R.lists(H, false);
+  if (E) {
+    L.iptr = E->iptr.as<int32_t>();
+    L.i_row = E->irow.as<int32_t>();
+    L.i_k = E->ik.as<int32_t>();
+  }
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * n_t);
   DevBuf dfld(sizeof(T) * static_cast<size_t>(n) * n_frames);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * 3);
@@ -871,8 +1310,12 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
       if (t >= sg.start[last]) store = rec->useg[0].as<T>() + (t - sg.start[last]) * n;
     }
     sampler.before(t);
-    launch_step<T>(H, 1, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t), q, hp,
-                   fp, field_is_vel, dtT, IR, store, nullptr);
+    if (E)
+      launch_step_ebe<T>(H, *E, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t),
+                         q, hp, fp, field_is_vel, dtT, IR);
+    else
+      launch_step<T>(H, 1, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t), q,
+                     hp, fp, field_is_vel, dtT, IR, store, nullptr);
     sampler.after(t);
     ++launches;
     if (H.nI > 0 && H.has_xinv) {
@@ -1169,6 +1612,7 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->sq = P.sq;
     H->sr = P.sr;
     H->alpha = P.alpha;
+    H->iface_host = P.iface;
     if (precision == BFE_PREC_DOUBLE) upload_plan<double>(*H, P);
     else upload_plan<float>(*H, P);
     *out = H.release();
@@ -1285,6 +1729,86 @@ bfe_status bfe_run_batched(bfe_handle h, int32_t n_t, int32_t n_src, const int32
   });
 }
 
+bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, int64_t n_el,
+                                    int32_t npe, const int32_t* el_nodes, const int32_t* el_tmpl,
+                                    const double* el_scale, int32_t n_tmpl, const double* tmpl,
+                                    double check_tol) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (!((npe == 3 && dpn >= 1 && dpn <= 3) || (npe == 4 && dpn == 3)))
+      throw std::invalid_argument("element operator: supported (nodes, DOF per node) are "
+                                  "(3, 1), (3, 2), (3, 3) and (4, 3)");
+    if (n_nodes < 1 || n_nodes * dpn != h->n)
+      throw std::invalid_argument("element operator: n must equal n_nodes * dof_per_node "
+                                  "(node-major numbering of every node's DOFs)");
+    if (n_el < 1 || n_el > INT32_MAX) throw std::invalid_argument("element count out of range");
+    if (n_tmpl < 1) throw std::invalid_argument("no element templates");
+    if (!el_nodes || !el_tmpl || !el_scale || !tmpl)
+      throw std::invalid_argument("NULL element operator array");
+    const int KE = npe * dpn;
+    for (int64_t e = 0; e < n_el; ++e) {
+      if (el_tmpl[e] < 0 || el_tmpl[e] >= n_tmpl)
+        throw std::invalid_argument("element template index out of range");
+      if (!std::isfinite(el_scale[e])) throw std::invalid_argument("non-finite element scale");
+      for (int l = 0; l < npe; ++l) {
+        const int32_t v = el_nodes[e * npe + l];
+        if (v < 0 || v >= n_nodes) throw std::invalid_argument("element node index out of range");
+      }
+    }
+    for (int64_t i = 0; i < static_cast<int64_t>(n_tmpl) * KE * KE; ++i)
+      if (!std::isfinite(tmpl[i])) throw std::invalid_argument("non-finite template entry");
+    bfe::EbeLayout Lay = bfe::build_ebe_layout(n_nodes, dpn, n_el, npe, el_nodes, el_tmpl,
+                                                n_tmpl, tmpl, kEbeSlotPad, 64);
+    std::vector<double> scale(el_scale, el_scale + n_el);
+    BFE_HIP(hipSetDevice(h->device));
+    auto E = std::make_unique<EbeOp>();
+    E->n_nodes = n_nodes;
+    E->n_el = n_el;
+    E->dpn = dpn;
+    E->npe = npe;
+    E->n_slot = Lay.n_slot_padded;
+    E->n_tmpl = n_tmpl;
+    E->n_cls = Lay.n_cls;
+    E->pattern = match_pattern(Lay);
+    if (E->pattern != 0 && !has_pattern_kernel(*E)) E->pattern = 0;
+    E->adj = upload(Lay.adj);
+    E->nodes = upload(Lay.nodes4);
+    if (E->pattern != 0) E->nbr = upload(Lay.nbr);
+    const int64_t nb = (n_nodes + kBlock - 1) / kBlock;
+    std::vector<int32_t> ks(h->iface_host.size());
+    for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
+    Grouped gi = group_by_block(nb, static_cast<int64_t>(kBlock) * dpn, h->iface_host, &ks, nullptr);
+    E->iptr = upload(gi.ptr);
+    E->irow = upload(gi.row);
+    E->ik = upload(gi.a);
+    const std::vector<double>& srows = Lay.srows;
+    const std::vector<double>& srows_c = Lay.srows_c;
+    double err;
+    if (h->prec == BFE_PREC_DOUBLE) {
+      E->scale = upload(scale);
+      E->srows = upload(srows);
+      if (E->pattern != 0) E->srows_c = upload(srows_c);
+      E->alpha = upload(h->alpha);
+      err = compare_ebe_with_csr<double>(*h, *E, true);
+      if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<double>(*h, *E, false));
+    } else {
+      E->scale = upload(cast_vec<float>(scale));
+      E->srows = upload(cast_vec<float>(srows));
+      if (E->pattern != 0) E->srows_c = upload(cast_vec<float>(srows_c));
+      E->alpha = upload(cast_vec<float>(h->alpha));
+      err = compare_ebe_with_csr<float>(*h, *E, true);
+      if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<float>(*h, *E, false));
+    }
+    E->check_err = err;
+    const double tol = check_tol > 0.0 ? check_tol : (h->prec == BFE_PREC_DOUBLE ? 1e-9 : 1e-3);
+    if (!(err <= tol))
+      throw std::invalid_argument("element operator does not reproduce K (relative step "
+                                  "difference " + std::to_string(err) + ")");
+    h->ebe = std::move(E);
+    h->use_ebe = 1;
+  });
+}
+
 bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
   return guarded([&] {
     if (h == nullptr) throw std::invalid_argument("NULL handle");
@@ -1296,6 +1820,21 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         break;
       case BFE_TUNE_XCD_REMAP:
         h->xcd_remap = value ? 1 : 0;
+        break;
+      case BFE_TUNE_EBE_GROUP:
+        if (value != 2 && value != 4 && value != 8)
+          throw std::invalid_argument("element-operator load group must be 2, 4 or 8");
+        h->ebe_group = value;
+        break;
+      case BFE_TUNE_EBE_KERNEL:
+        h->ebe_kernel = value ? 1 : 0;
+        break;
+      case BFE_TUNE_OPERATOR:
+        if (value != BFE_OPERATOR_CSR && value != BFE_OPERATOR_EBE)
+          throw std::invalid_argument("operator must be BFE_OPERATOR_CSR or BFE_OPERATOR_EBE");
+        if (value == BFE_OPERATOR_EBE && !h->ebe)
+          throw std::invalid_argument("no element operator attached");
+        h->use_ebe = value == BFE_OPERATOR_EBE ? 1 : 0;
         break;
       default:
         throw std::invalid_argument("unknown tuning key");
@@ -1325,6 +1864,12 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
     I.precision = h->prec;
     I.device = h->device;
     I.device_bytes = h->device_bytes();
+    I.operator_kind = (h->ebe && h->use_ebe) ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
+    I.ebe_slots = h->ebe ? h->ebe->n_slot : 0;
+    I.ebe_templates = h->ebe ? h->ebe->n_tmpl : 0;
+    I.ebe_elements = h->ebe ? h->ebe->n_el : 0;
+    I.ebe_check = h->ebe ? h->ebe->check_err : 0.0;
+    I.ebe_pattern = h->ebe ? h->ebe->pattern : 0;
     *info = I;
   });
 }

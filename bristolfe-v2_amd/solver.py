@@ -54,15 +54,31 @@ def lumped_mass(M) -> np.ndarray:
 
 
 def make_plan(K, C, M, dt, solver_mode="vel at curent time step", solver_precision="double",
-              device=0) -> _lib.Plan:
+              device=0, element_op=None, operator="auto") -> _lib.Plan:
+    """Plan of the march. With an ElementOperator (bristolfe_v2_amd.ebe) the
+    single-source march uses the element-by-element step: operator "ebe"
+    requires it, "auto" falls back to the CSR step when the library rejects
+    it, "csr" ignores it."""
     if K.shape[0] != K.shape[1]:
         raise ValueError("K must be square matrix")
     if C.shape[0] != C.shape[1]:
         raise ValueError("C must be square matrix")
     if M.shape[0] != M.shape[1]:
         raise ValueError("M must be square matrix")
-    return _lib.Plan.from_scipy(K, lumped_mass(M), C, dt, _mode(solver_mode),
+    if operator not in ("auto", "csr", "ebe"):
+        raise ValueError(f"unknown stiffness operator {operator!r}")
+    if operator == "ebe" and element_op is None:
+        raise ValueError("operator 'ebe' needs an element operator for this model")
+    plan = _lib.Plan.from_scipy(K, lumped_mass(M), C, dt, _mode(solver_mode),
                                 _precision(solver_precision), device)
+    if element_op is not None and operator != "csr":
+        try:
+            plan.set_element_operator(element_op)
+        except _lib.BfeError as e:
+            if operator == "ebe" or e.status != _lib.BFE_ERR_ARG:
+                plan.close()
+                raise
+    return plan
 
 
 def field_frames(n_t: int, every) -> np.ndarray:

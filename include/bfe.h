@@ -67,6 +67,12 @@ typedef struct bfe_info {
   int32_t precision;
   int32_t device;
   int64_t device_bytes;  /* operator + coefficient bytes on the device   */
+  int32_t operator_kind; /* BFE_OPERATOR_CSR or BFE_OPERATOR_EBE (march) */
+  int32_t ebe_slots;     /* (template, vertex) slots of the element operator */
+  int32_t ebe_templates; /* element templates                            */
+  int32_t ebe_pattern;   /* mesh pattern of the register kernel, 0: none  */
+  int64_t ebe_elements;  /* elements of the element operator             */
+  double ebe_check;      /* set-up check: relative step difference to CSR */
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
@@ -200,7 +206,45 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  */
 #define BFE_TUNE_MULTI_ROWS_PER_BLOCK 1
 #define BFE_TUNE_XCD_REMAP 2
+/* operator of the single-source march once an element operator is attached
+ * (default: the element operator) */
+#define BFE_TUNE_OPERATOR 3
+#define BFE_OPERATOR_CSR 0
+#define BFE_OPERATOR_EBE 1
+/* slots of the element operator whose loads are issued together in the
+ * generic gather kernel (2, 4 or 8; default 8), and whether a mesh-pattern
+ * kernel is used when the mesh matches one (0 or 1; default 1) */
+#define BFE_TUNE_EBE_GROUP 4
+#define BFE_TUNE_EBE_KERNEL 5
 bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value);
+
+/*
+ * bfe_set_element_operator — attach an element-by-element form of K to the
+ * handle for the single-source march (bfe_run). Replaces the product K u of
+ * v6:185 (B = X^-1 (2I - dt^2 M^-1 K), v6:137) by a gather over the
+ * elements around each node, with the element matrices of
+ * fn_build_global_matrices_v5.m:105-128 stored as a few templates:
+ *   K = sum_e el_scale[e] * P_e' tmpl[el_tmpl[e]] P_e
+ * el_scale carries the absorbing-layer stiffness reduction (v5:116) and any
+ * per-element factor; P_e maps local DOF (vertex l, component c), index
+ * l*dpn + c, to global DOF el_nodes[e*npe + l]*dpn + c. The handle's DOFs
+ * must be numbered node-major with dpn DOFs on every node (n = n_nodes*dpn).
+ *   el_nodes (n_el x npe, 0-based), el_tmpl (n_el), el_scale (n_el)
+ *   tmpl (n_tmpl x (npe*dpn)^2, row-major)
+ * Shapes: npe = 3 with dpn 1, 2 or 3 (triangles), npe = 4 with dpn = 3 (tets).
+ * At most one element of each template may have a given node as its local
+ * vertex l (true of structured meshes), and at most 64 (template, vertex)
+ * pairs may occur. Before accepting it, the library marches one step of both
+ * operators from the same random state and rejects the element operator
+ * (BFE_ERR_ARG) if the results differ by more than check_tol relative
+ * (<= 0: 1e-9 in double, 1e-3 in single precision). The CSR operator stays
+ * in use for bfe_run_batched and the checkpointed/adjoint runs.
+ * No counterpart in the reference (which always forms sparse B).
+ */
+bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, int64_t n_el,
+                                    int32_t npe, const int32_t* el_nodes, const int32_t* el_tmpl,
+                                    const double* el_scale, int32_t n_tmpl, const double* tmpl,
+                                    double check_tol);
 
 /*
  * bfe_set_kernel_timing — sample the duration of the main step kernel.

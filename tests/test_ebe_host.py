@@ -1,0 +1,78 @@
+"""Element-by-element operator, host side: the templates and scales that
+``element_operator`` finds reassemble the global K of the CSR path, and
+models that do not have that form are refused."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import PKG, assemble, coupled_block, solid_plate
+
+
+def _ebe():
+    from bristolfe_v2_amd import ebe
+    return ebe
+
+
+def _reassemble(op, n):
+    npe = op.el_nodes.shape[1]
+    dpn = op.dpn
+    ke = npe * dpn
+    dof = (op.el_nodes[:, :, None].astype(np.int64) * dpn + np.arange(dpn)).reshape(-1, ke)
+    vals = op.tmpl[op.el_tmpl] * op.el_scale[:, None, None]
+    r = np.broadcast_to(dof[:, :, None], vals.shape)
+    c = np.broadcast_to(dof[:, None, :], vals.shape)
+    return sp.csr_matrix((vals.ravel(), (r.ravel(), c.ravel())), shape=(n, n))
+
+
+@pytest.mark.parametrize("case", ["plate", "plate_abs", "tets", "plate_all_dofs"])
+def test_templates_reassemble_K(case):
+    m = PKG.mesh
+    if case.startswith("plate"):
+        mod, matls = solid_plate(2e-3, 1e-3, 0.1e-3)
+        opts = {"dof_to_use": [] if case == "plate_all_dofs" else [1, 2], "damping_power_law": 2}
+        if case == "plate_abs":
+            inner = np.array([[0.5e-3, 0.25e-3], [1.5e-3, 0.25e-3], [1.5e-3, 5e-3], [0.5e-3, 5e-3]])
+            mod = m.fn_add_absorbing_layer(mod, inner, 0.25e-3)
+    else:
+        from bristolfe_v2_amd import configs
+        c = configs.cube_3d(el_size=1e-3, n_t=10)
+        mod, matls, opts = c["mod"], c["matls"], c["fe_options"]
+    K, C, M, gl = assemble(mod, matls, opts)
+    op = _ebe().element_operator(mod, matls, opts, gl, K.shape[0])
+    assert op is not None
+    # CPE3 with every DOF keeps uz (anti-plane shear): 3 DOFs per node
+    assert op.dpn == {"plate_all_dofs": 3, "tets": 3}.get(case, 2)
+    npe = 4 if case == "tets" else 3
+    assert op.el_nodes.shape == (mod["els"].shape[0], npe)
+    assert op.tmpl.shape[0] == (6 if case == "tets" else 2)
+    K2 = _reassemble(op, K.shape[0])
+    assert abs(K2 - K).max() <= 1e-12 * abs(K).max()
+    # the absorbing scale is exactly 1 outside the layer
+    a = np.asarray(mod["el_abs_i"])
+    assert np.all(op.el_scale[a == 0] == 1.0)
+    if case == "plate_abs":
+        assert op.el_scale.min() < 0.05
+
+
+def test_models_without_template_form_are_refused():
+    mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.25e-3)
+    opts = {"dof_to_use": [1, 2, 4]}
+    K, C, M, gl = assemble(mod, matls, opts)
+    assert _ebe().element_operator(mod, matls, opts, gl, K.shape[0]) is None
+    # an irregular (perturbed) mesh has one template per element shape: too many
+    mod, matls = solid_plate(1e-3, 1e-3, 0.1e-3)
+    rng = np.random.default_rng(0)
+    mod["nds"] = mod["nds"] + rng.uniform(-1e-6, 1e-6, mod["nds"].shape)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2]})
+    assert _ebe().element_operator(mod, matls, {"dof_to_use": [1, 2]}, gl, K.shape[0]) is None
+
+
+def test_bytes_per_dof_formulae():
+    from bristolfe_v2_amd import configs
+    c = configs.cube_3d(el_size=1e-3, n_t=10)
+    K, gl = c["K"], c["gl_lookup"]
+    op = _ebe().element_operator(c["mod"], c["matls"], c["fe_options"], gl, K.shape[0])
+    n = K.shape[0]
+    b = _ebe().ebe_bytes_per_dof(op, n, True)
+    assert np.isclose(b, op.n_el / n * 25 + 24 + 16)
+    assert _ebe().ebe_stored_bytes_per_dof(op, n, 24, True) > b
