@@ -355,16 +355,24 @@ def main():
     wall = time.perf_counter() - wall0
     loop_s = float(np.mean(loops)) / 1e3
     value = n * steps_done / loop_s / 1e9
+    pattern = use_ebe and info["ebe_pattern"] != 0
+    kbase = "k_step_ebe_pat" if pattern else ("k_step_ebe" if use_ebe else "k_step")
     if use_ebe:
         bpd = ebe.ebe_bytes_per_dof(op, n, info["damped"])
-        kname = (f"k_step_ebe<double,{'true' if info['damped'] else 'false'},{op.el_nodes.shape[1]},"
-                 f"{op.dpn}> (element-by-element gather + central-difference update)")
+        dmp = "true" if info["damped"] else "false"
+        if pattern:
+            pname = {1: "PatKuhn24", 2: "PatTriRect", 3: "PatTriIso"}[info["ebe_pattern"]]
+            kname = (f"k_step_ebe_pat<double,{dmp},{pname},{op.dpn}> (element-by-element, "
+                     "neighbour u in registers, + central-difference update)")
+        else:
+            kname = (f"k_step_ebe<double,{dmp},{op.el_nodes.shape[1]},{op.dpn},8> "
+                     "(element-by-element gather + central-difference update)")
     else:
         bpd = algorithmic_bytes_per_dof(info["nnz_k"], n, info["damped"])
         kname = "k_step<double,true> (SELL-64 SpMV + central-difference update)"
     kern_avg_s = kern / max(samples, 1) / 1e3
     achieved = bpd * n / kern_avg_s / 1e9 if samples else None
-    traffic = read_traffic(args.workload, "k_step_ebe" if use_ebe else "k_step")
+    traffic = read_traffic(args.workload, kbase)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -395,7 +403,8 @@ def main():
         out["config"]["operator"] = (f"element-by-element: {op.n_el} elements, "
                                      f"{info['ebe_templates']} templates, {info['ebe_slots']} slots")
         out["roofline"]["stored_bytes_per_dof_step"] = round(
-            ebe.ebe_stored_bytes_per_dof(op, n, info["ebe_slots"], info["damped"]), 2)
+            ebe.ebe_stored_bytes_per_dof(op, n, info["ebe_slots"], info["damped"],
+                                         info["ebe_classes"] if pattern else 0), 2)
         # the CSR operator of the same plan, one march, for comparison
         plan.set_tuning(3, 0)
         plan.set_kernel_timing(args.timing_every)

@@ -43,7 +43,8 @@ class BfeInfo(C.Structure):
         ("damped", C.c_int32), ("mode", C.c_int32), ("precision", C.c_int32),
         ("device", C.c_int32), ("device_bytes", C.c_int64), ("operator_kind", C.c_int32),
         ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
-        ("ebe_elements", C.c_int64), ("ebe_check", C.c_double),
+        ("ebe_elements", C.c_int64), ("ebe_check", C.c_double), ("ebe_classes", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 

@@ -88,6 +88,7 @@ EbeLayout build_ebe_layout(int64_t n_nodes, int32_t dpn, int64_t n_el, int32_t n
     for (size_t i = 0; i < off.size(); ++i)
       cls0[i] = static_cast<int32_t>(std::lower_bound(dist.begin(), dist.end(), off[i]) - dist.begin());
     L.self_cls = static_cast<int32_t>(std::lower_bound(dist.begin(), dist.end(), 0) - dist.begin());
+    L.cls_off = dist;
     // every node: the vertices of one class must be one node
     nbr.assign(static_cast<size_t>(ncls) * n_nodes, -1);
     std::atomic<bool> bad{false};
@@ -109,6 +110,7 @@ EbeLayout build_ebe_layout(int64_t n_nodes, int32_t dpn, int64_t n_el, int32_t n
       cls0.clear();
       nbr.clear();
       L.self_cls = -1;
+      L.cls_off.clear();
     }
   }
 
@@ -143,8 +145,21 @@ EbeLayout build_ebe_layout(int64_t n_nodes, int32_t dpn, int64_t n_el, int32_t n
         L.srows[(static_cast<size_t>(kn) * dpn + c) * KE + j] =
             tmpl[(static_cast<size_t>(tau) * KE + l * dpn + c) * KE + j];
   }
-  if (ncls > 0) {
+  if (ncls > 0 && ns <= 31) {  // the per-node slot mask has 31 slot bits
     L.n_cls = ncls;
+    L.present.assign(n_nodes, 0u);
+    par_nodes(n_nodes, [&](int64_t v0, int64_t v1) {
+      for (int64_t v = v0; v < v1; ++v) {
+        uint32_t m = 0;
+        for (int32_t kn = 0; kn < ns; ++kn)
+          if (L.adj[static_cast<size_t>(kn) * n_nodes + v] >= 0) m |= 1u << kn;
+        for (int32_t j = 0; j < ncls; ++j) {
+          const int32_t w = nbr[static_cast<size_t>(j) * n_nodes + v];
+          if (w >= 0 && w != v + L.cls_off[j]) m |= 1u << 31;
+        }
+        L.present[v] = m;
+      }
+    });
     L.nbr = std::move(nbr);
     L.cls.resize(static_cast<size_t>(ns) * npe);
     L.srows_c.assign(L.srows.size(), 0.0);

@@ -35,6 +35,10 @@ struct EbeLayout {
   // neighbour classes (empty when the mesh has no consistent class structure)
   int32_t n_cls = 0;
   int32_t self_cls = -1;         // class of the node itself
+  std::vector<int64_t> cls_off;  // n_cls: index offset w - v of each class at the reference node
+  // per node: bit k set if slot k is occupied; bit 31 set if some neighbour
+  // is not at its class offset (the node must read nbr)
+  std::vector<uint32_t> present;
   std::vector<int32_t> cls;      // n_slot x npe, class of canonical vertex b of slot k
   std::vector<int32_t> nbr;      // n_cls x n_nodes, node or -1
   std::vector<double> srows_c;   // n_slot_padded x dpn x npe*dpn (canonical vertex order)

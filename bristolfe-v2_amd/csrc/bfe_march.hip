@@ -380,24 +380,26 @@ __device__ __forceinline__ void ebe_load_records(const int32_t* e0, const int4* 
 }
 
 // The update of node v's DPN rows from acc = (K u1) on those rows, with the
-// forcing, interface, field and monitor handling of k_step.
+// forcing, interface, field and monitor handling of k_step. alpha and beta
+// are per DOF, or per node when coef_per_node is set.
 template <typename T, bool DAMPED, int DPN>
 __device__ __forceinline__ void ebe_finish(
     int64_t v, int64_t lb, const T* acc, const T* __restrict__ alpha, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
     T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    const IfaceRows<T>& I) {
+    const IfaceRows<T>& I, int32_t coef_per_node) {
   const int64_t r0 = v * DPN;
   T a[DPN], b[DPN], un[DPN];
 #pragma unroll
   for (int c = 0; c < DPN; ++c) {
     const int64_t r = r0 + c;
+    const int64_t ci = coef_per_node ? v : r;  // alpha/beta per node when a node's DOFs share them
     a[c] = u1[r];
     b[c] = u2[r];
-    const T ka = alpha[r] * acc[c];
+    const T ka = alpha[ci] * acc[c];
     if constexpr (DAMPED) {
-      const T be = beta[r];
+      const T be = beta[ci];
       un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
     } else {
       un[c] = (T(2) * a[c] - b[c]) - ka;
@@ -474,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe(
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
     T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    IfaceRows<T> I) {
+    IfaceRows<T> I, int32_t coef_per_node) {
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t v = lb * kBlock + threadIdx.x;
   if (v >= n_nodes) return;
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe(
   for (int c = 0; c < DPN; ++c) acc[c] = T(0);
   ebe_gather<T, NPE, DPN, G>(n_nodes, v, n_slot, adj, el_nodes, el_scale, srows, u1, acc);
   ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
-                             hist_prev, fld_prev, fld_is_vel, dt, I);
+                             hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,30 +513,60 @@ struct PatTriIso {  // near-equilateral triangles
                                           {2, 3, 6}, {2, 4, 5}, {2, 5, 6}};
 };
 
-// Element-by-element step on a mesh pattern: the NC neighbours' u1 values are
-// gathered once (neighbour table nbr, class-major) and kept in registers;
-// every slot then needs only its table entry and element scale, and its
-// template rows (canonical vertex order) come through the scalar cache.
+// a if bit == 1, b if bit == 0, by bit masking (no lane-mask predicate).
+__device__ __forceinline__ double pick_bits(double a, double b, uint32_t bit) {
+  const uint64_t m = 0ull - static_cast<uint64_t>(bit);
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<uint64_t>(__double_as_longlong(a)) & m) |
+      (static_cast<uint64_t>(__double_as_longlong(b)) & ~m)));
+}
+__device__ __forceinline__ float pick_bits(float a, float b, uint32_t bit) {
+  const uint32_t m = 0u - bit;
+  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+// Class offsets (w - v) of a pattern's neighbours on regular nodes.
+template <int NC>
+struct ClassOffsets {
+  int32_t off[NC];
+};
+
+// Element-by-element step on a mesh pattern (at most 128 VGPRs, so that four
+// wavefronts share a SIMD: the kernel is bound by load latency more than by
+// its FP64 FMAs). The NC neighbours' u1 values are
+// gathered once and kept in registers; on a regular node neighbour j is node
+// v + off[j] (irregular nodes, flagged in bit 31 of pmask, read the class-major
+// neighbour table nbr). pmask bit k marks an occupied slot, smask bit k a slot
+// whose element scale differs from 1: only those read the slot table and the
+// scale. Template rows (canonical vertex order) come through the scalar cache.
 template <typename T, bool DAMPED, typename PT, int DPN>
-__global__ __launch_bounds__(kBlock) void k_step_ebe_pat(
-    int64_t n_nodes, int64_t nblk, const int32_t* __restrict__ nbr,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
+    int64_t n_nodes, int64_t nblk, ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
+    const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
     const T* __restrict__ srows, const T* __restrict__ alpha, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
     T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    IfaceRows<T> I) {
+    IfaceRows<T> I, int32_t coef_per_node) {
   constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t v = lb * kBlock + threadIdx.x;
   if (v >= n_nodes) return;
+  const uint32_t pm = pmask[v], sm = smask[v];
+  const int64_t ib = pm >> 31;  // 1: irregular node
+  // Loads are issued on every lane without branching (a divergent branch
+  // around a load serialises the wave on its latency); lanes that do not
+  // need a table entry read entry 0 instead, one cached line for the wave.
   T x[NC][DPN];
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     int64_t w = v;
     if (j != PT::SELF) {
-      const int32_t wj = nbr[static_cast<int64_t>(j) * n_nodes + v];
-      w = wj < 0 ? v : wj;  // an absent neighbour is used only by empty slots
+      const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
+      const int64_t wr = v + co.off[j];  // outside [0, n): absent, used only by empty slots
+      const int64_t wi = wj < 0 ? v : wj;
+      w = ib * wi + (1 - ib) * (wr < 0 || wr >= n_nodes ? v : wr);
     }
 #pragma unroll
     for (int d = 0; d < DPN; ++d) x[j][d] = u1[w * DPN + d];
@@ -544,8 +576,12 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe_pat(
   for (int c = 0; c < DPN; ++c) acc[c] = T(0);
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
-    const int32_t e0 = adj[static_cast<int64_t>(k) * n_nodes + v];
-    const T s = el_scale[e0 < 0 ? 0 : e0];
+    // predicates as integer lane values (0 or 1) and bit masks: kept per
+    // lane in VGPRs, where 24 slots' worth of boolean lane masks would
+    // exhaust the scalar registers
+    const uint32_t sb = (sm >> k) & 1u;
+    const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * sb];
+    const T s = pick_bits(el_scale[static_cast<int64_t>(e) * sb], T(1), sb);
     const T* R = srows + k * (DPN * KE);
     T y[DPN];
 #pragma unroll
@@ -556,11 +592,12 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe_pat(
       for (int d = 0; d < DPN; ++d)
 #pragma unroll
         for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
+    const uint32_t ob = (pm >> k) & 1u;
 #pragma unroll
-    for (int c = 0; c < DPN; ++c) acc[c] += e0 < 0 ? T(0) : s * y[c];
+    for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
   }
   ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
-                             hist_prev, fld_prev, fld_is_vel, dt, I);
+                             hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
 }
 
 // Pattern id of a layout's class table, 0 if none matches.
@@ -718,12 +755,15 @@ struct EbeOp {
   int64_t n_nodes = 0, n_el = 0;
   int32_t dpn = 0, npe = 0, n_slot = 0, n_tmpl = 0, n_cls = 0;
   int32_t pattern = 0;  // mesh pattern of k_step_ebe_pat, 0: none
-  DevBuf adj, nodes, scale, srows, alpha, nbr, srows_c;
+  int32_t coef_per_node = 0;  // alpha (and beta) stored per node
+  std::vector<int64_t> cls_off;
+  DevBuf adj, nodes, scale, srows, alpha, beta, nbr, srows_c, pmask, smask;
   DevBuf iptr, irow, ik;  // interface rows grouped per k_step_ebe workgroup
   double check_err = 0.0;  // largest relative difference to the CSR step at set-up
   int64_t bytes() const {
     int64_t s = 0;
-    for (const DevBuf* b : {&adj, &nodes, &scale, &srows, &alpha, &nbr, &srows_c, &iptr, &irow, &ik})
+    for (const DevBuf* b : {&adj, &nodes, &scale, &srows, &alpha, &beta, &nbr, &srows_c, &pmask,
+                            &smask, &iptr, &irow, &ik})
       s += static_cast<int64_t>(b->bytes);
     return s;
   }
@@ -758,6 +798,7 @@ struct bfe_plan_s {
   std::map<int32_t, std::shared_ptr<void>> adj_records;  // AdjointRecord per slot
   std::shared_ptr<void> grad_elems;  // GradElements
   std::vector<int32_t> iface_host;   // interface rows (global), for regrouping
+  std::vector<double> beta_host;     // beta per row (damped plans), for the element operator
   std::unique_ptr<EbeOp> ebe;        // optional element-by-element operator
   int32_t use_ebe = 1;               // single-source march uses ebe when set
   int32_t ebe_group = 8;             // slots per load group of k_step_ebe (2, 4 or 8)
@@ -1085,8 +1126,8 @@ void launch_ebe_group(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T
   hipLaunchKernelGGL((k_step_ebe<T, D, NPE, DPN, G>), dim3(nblk), dim3(kBlock), 0, H.stream,
                      E.n_nodes, nblk, E.n_slot, E.adj.as<int32_t>(), E.nodes.as<int4>(),
                      E.scale.as<T>(), E.srows.as<T>(), E.alpha.as<T>(),
-                     H.damped ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q,
-                     hp, fp, fld_is_vel, dt, IR);
+                     H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q,
+                     hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
 template <typename T, bool D, int NPE, int DPN>
@@ -1121,10 +1162,13 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* 
                     T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
                     int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
                     const IfaceRows<T>& IR) {
+  ClassOffsets<PT::NC> co;
+  for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(E.cls_off[j]);
   hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN>), dim3(nblk), dim3(kBlock), 0, H.stream,
-                     E.n_nodes, nblk, E.nbr.as<int32_t>(), E.adj.as<int32_t>(), E.scale.as<T>(),
-                     E.srows_c.as<T>(), E.alpha.as<T>(), H.damped ? H.beta.as<T>() : nullptr, u1,
-                     u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR);
+                     E.n_nodes, nblk, co, E.pmask.as<uint32_t>(), E.smask.as<uint32_t>(),
+                     E.nbr.as<int32_t>(), E.adj.as<int32_t>(), E.scale.as<T>(), E.srows_c.as<T>(),
+                     E.alpha.as<T>(), H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha,
+                     F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
 template <typename T, bool D>
@@ -1613,6 +1657,7 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->sr = P.sr;
     H->alpha = P.alpha;
     H->iface_host = P.iface;
+    if (P.damped) H->beta_host = P.beta;
     if (precision == BFE_PREC_DOUBLE) upload_plan<double>(*H, P);
     else upload_plan<float>(*H, P);
     *out = H.release();
@@ -1773,7 +1818,43 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
     if (E->pattern != 0 && !has_pattern_kernel(*E)) E->pattern = 0;
     E->adj = upload(Lay.adj);
     E->nodes = upload(Lay.nodes4);
-    if (E->pattern != 0) E->nbr = upload(Lay.nbr);
+    if (E->pattern != 0) {
+      E->nbr = upload(Lay.nbr);
+      E->cls_off = Lay.cls_off;
+      // slots whose element scale is not exactly 1 (absorbing layer)
+      std::vector<uint32_t> sm(n_nodes, 0u);
+      for (int32_t k = 0; k < Lay.n_slot; ++k)
+        for (int64_t v = 0; v < n_nodes; ++v) {
+          const int32_t e = Lay.adj[static_cast<size_t>(k) * n_nodes + v];
+          if (e >= 0 && el_scale[e] != 1.0) sm[v] |= 1u << k;
+        }
+      E->pmask = upload(Lay.present);
+      E->smask = upload(sm);
+    }
+    // alpha and beta per node when every node's DOFs share them
+    bool per_node = true;
+    const std::vector<double>& al = h->alpha;
+    for (int64_t v = 0; v < n_nodes && per_node; ++v)
+      for (int c = 1; c < dpn; ++c)
+        if (al[v * dpn + c] != al[v * dpn]) { per_node = false; break; }
+    if (per_node && h->damped) {
+      for (int64_t v = 0; v < n_nodes && per_node; ++v)
+        for (int c = 1; c < dpn; ++c)
+          if (h->beta_host[v * dpn + c] != h->beta_host[v * dpn]) { per_node = false; break; }
+    }
+    E->coef_per_node = per_node ? 1 : 0;
+    std::vector<double> alpha_e, beta_e;
+    if (per_node) {
+      alpha_e.resize(n_nodes);
+      for (int64_t v = 0; v < n_nodes; ++v) alpha_e[v] = al[v * dpn];
+      if (h->damped) {
+        beta_e.resize(n_nodes);
+        for (int64_t v = 0; v < n_nodes; ++v) beta_e[v] = h->beta_host[v * dpn];
+      }
+    } else {
+      alpha_e = al;
+      if (h->damped) beta_e = h->beta_host;
+    }
     const int64_t nb = (n_nodes + kBlock - 1) / kBlock;
     std::vector<int32_t> ks(h->iface_host.size());
     for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
@@ -1788,14 +1869,16 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
       E->scale = upload(scale);
       E->srows = upload(srows);
       if (E->pattern != 0) E->srows_c = upload(srows_c);
-      E->alpha = upload(h->alpha);
+      E->alpha = upload(alpha_e);
+      if (h->damped) E->beta = upload(beta_e);
       err = compare_ebe_with_csr<double>(*h, *E, true);
       if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<double>(*h, *E, false));
     } else {
       E->scale = upload(cast_vec<float>(scale));
       E->srows = upload(cast_vec<float>(srows));
       if (E->pattern != 0) E->srows_c = upload(cast_vec<float>(srows_c));
-      E->alpha = upload(cast_vec<float>(h->alpha));
+      E->alpha = upload(cast_vec<float>(alpha_e));
+      if (h->damped) E->beta = upload(cast_vec<float>(beta_e));
       err = compare_ebe_with_csr<float>(*h, *E, true);
       if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<float>(*h, *E, false));
     }
@@ -1870,6 +1953,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
     I.ebe_elements = h->ebe ? h->ebe->n_el : 0;
     I.ebe_check = h->ebe ? h->ebe->check_err : 0.0;
     I.ebe_pattern = h->ebe ? h->ebe->pattern : 0;
+    I.ebe_classes = h->ebe ? h->ebe->n_cls : 0;
     *info = I;
   });
 }

@@ -139,8 +139,16 @@ def ebe_bytes_per_dof(op: ElementOperator, n: int, damped: bool) -> float:
     return op.n_el / n * (4 * npe + 1 + 8) + 24 + 8 * (2 if damped else 1)
 
 
-def ebe_stored_bytes_per_dof(op: ElementOperator, n: int, n_slot: int, damped: bool) -> float:
-    """Bytes per DOF-step this implementation streams if every element record
-    is read once: the slot-major node-to-element table (4 n_slot per node),
-    padded node quads (16) and the scale (8) per element, and the vectors."""
-    return (4.0 * n_slot * op.n_nodes + 24.0 * op.n_el) / n + 24 + 8 * (2 if damped else 1)
+def ebe_stored_bytes_per_dof(op: ElementOperator, n: int, n_slot: int, damped: bool,
+                             n_classes: int = 0) -> float:
+    """Bytes per DOF-step this implementation streams if every record is read
+    once: the slot-major node-to-element table (4 n_slot per node), and either
+    (mesh-pattern kernel, n_classes > 0) the neighbour table (4 per neighbour
+    class other than the node itself) and the element scale (8), or (generic
+    kernel) the padded node quads (16) and the scale (8) per element; plus the
+    vectors and coefficient streams."""
+    if n_classes > 0:
+        per_node, per_el = 4.0 * n_slot + 4.0 * (n_classes - 1), 8.0
+    else:
+        per_node, per_el = 4.0 * n_slot, 24.0
+    return (per_node * op.n_nodes + per_el * op.n_el) / n + 24 + 8 * (2 if damped else 1)

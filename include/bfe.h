@@ -73,6 +73,8 @@ typedef struct bfe_info {
   int32_t ebe_pattern;   /* mesh pattern of the register kernel, 0: none  */
   int64_t ebe_elements;  /* elements of the element operator             */
   double ebe_check;      /* set-up check: relative step difference to CSR */
+  int32_t ebe_classes;   /* neighbour classes of the mesh pattern         */
+  int32_t reserved1;
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
