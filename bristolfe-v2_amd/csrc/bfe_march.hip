@@ -544,17 +544,45 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     int64_t n_nodes, int64_t nblk, ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
     const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
-    const T* __restrict__ srows, const T* __restrict__ alpha, const T* __restrict__ beta,
-    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
-    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
-    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    IfaceRows<T> I, int32_t coef_per_node) {
+    const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
+    const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
+    T* __restrict__ u, BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F,
+    int32_t F_rows, int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
+    int32_t fld_is_vel, T dt, IfaceRows<T> I, int32_t coef_per_node) {
   constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t v = lb * kBlock + threadIdx.x;
   if (v >= n_nodes) return;
   const uint32_t pm = pmask[v], sm = smask[v];
   const int64_t ib = pm >> 31;  // 1: irregular node
+  constexpr uint32_t kAll = (1u << NS) - 1u;
+  if (__all((pm & kAll) == kAll && sm == 0u)) {
+    // every lane's elements are all present with scale 1 (the interior): K u
+    // is the fixed stencil sum_j B_j u_j, B_j the sum of the template blocks
+    // of neighbour class j (NC DPN x DPN blocks instead of NS template rows)
+    T acc[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      int64_t w = v;
+      if (j != PT::SELF) {
+        const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
+        const int64_t wr = v + co.off[j];
+        const int64_t wi = wj < 0 ? v : wj;
+        w = ib * wi + (1 - ib) * (wr < 0 || wr >= n_nodes ? v : wr);
+      }
+#pragma unroll
+      for (int d = 0; d < DPN; ++d) {
+        const T xd = u1[w * DPN + d];
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) acc[c] += stencil[(j * DPN + c) * DPN + d] * xd;
+      }
+    }
+    ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
+                               hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+    return;
+  }
   // Loads are issued on every lane without branching (a divergent branch
   // around a load serialises the wave on its latency); lanes that do not
   // need a table entry read entry 0 instead, one cached line for the wave.
@@ -757,13 +785,13 @@ struct EbeOp {
   int32_t pattern = 0;  // mesh pattern of k_step_ebe_pat, 0: none
   int32_t coef_per_node = 0;  // alpha (and beta) stored per node
   std::vector<int64_t> cls_off;
-  DevBuf adj, nodes, scale, srows, alpha, beta, nbr, srows_c, pmask, smask;
+  DevBuf adj, nodes, scale, srows, alpha, beta, nbr, srows_c, pmask, smask, stencil;
   DevBuf iptr, irow, ik;  // interface rows grouped per k_step_ebe workgroup
   double check_err = 0.0;  // largest relative difference to the CSR step at set-up
   int64_t bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&adj, &nodes, &scale, &srows, &alpha, &beta, &nbr, &srows_c, &pmask,
-                            &smask, &iptr, &irow, &ik})
+                            &smask, &stencil, &iptr, &irow, &ik})
       s += static_cast<int64_t>(b->bytes);
     return s;
   }
@@ -1167,7 +1195,7 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* 
   hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN>), dim3(nblk), dim3(kBlock), 0, H.stream,
                      E.n_nodes, nblk, co, E.pmask.as<uint32_t>(), E.smask.as<uint32_t>(),
                      E.nbr.as<int32_t>(), E.adj.as<int32_t>(), E.scale.as<T>(), E.srows_c.as<T>(),
-                     E.alpha.as<T>(), H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha,
+                     E.stencil.as<T>(), E.alpha.as<T>(), H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha,
                      F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
@@ -1831,6 +1859,20 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
       E->pmask = upload(Lay.present);
       E->smask = upload(sm);
     }
+    // summed stencil of the pattern: B_j = sum over (slot, vertex) of class j
+    std::vector<double> stencil;
+    if (E->pattern != 0) {
+      const int ke = npe * dpn;
+      stencil.assign(static_cast<size_t>(Lay.n_cls) * dpn * dpn, 0.0);
+      for (int32_t k = 0; k < Lay.n_slot; ++k)
+        for (int b = 0; b < npe; ++b) {
+          const int32_t j = Lay.cls[k * npe + b];
+          for (int c = 0; c < dpn; ++c)
+            for (int d = 0; d < dpn; ++d)
+              stencil[(static_cast<size_t>(j) * dpn + c) * dpn + d] +=
+                  Lay.srows_c[(static_cast<size_t>(k) * dpn + c) * ke + b * dpn + d];
+        }
+    }
     // alpha and beta per node when every node's DOFs share them
     bool per_node = true;
     const std::vector<double>& al = h->alpha;
@@ -1868,7 +1910,10 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
     if (h->prec == BFE_PREC_DOUBLE) {
       E->scale = upload(scale);
       E->srows = upload(srows);
-      if (E->pattern != 0) E->srows_c = upload(srows_c);
+      if (E->pattern != 0) {
+        E->srows_c = upload(srows_c);
+        E->stencil = upload(stencil);
+      }
       E->alpha = upload(alpha_e);
       if (h->damped) E->beta = upload(beta_e);
       err = compare_ebe_with_csr<double>(*h, *E, true);
@@ -1876,7 +1921,10 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
     } else {
       E->scale = upload(cast_vec<float>(scale));
       E->srows = upload(cast_vec<float>(srows));
-      if (E->pattern != 0) E->srows_c = upload(cast_vec<float>(srows_c));
+      if (E->pattern != 0) {
+        E->srows_c = upload(cast_vec<float>(srows_c));
+        E->stencil = upload(cast_vec<float>(stencil));
+      }
       E->alpha = upload(cast_vec<float>(alpha_e));
       if (h->damped) E->beta = upload(cast_vec<float>(beta_e));
       err = compare_ebe_with_csr<float>(*h, *E, true);
