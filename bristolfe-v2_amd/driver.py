@@ -24,6 +24,7 @@ import numpy as np
 from .assembly import fn_build_global_matrices_v5
 from .field_output import fn_get_field_output
 from .options import BRISTOLFE_DEFAULTS, ENTRY_POINT_DEFAULTS, set_default_fields
+from .ebe import element_operator
 from .solver import fn_explicit_dynamic_solver_v6, make_plan
 
 
@@ -79,6 +80,7 @@ def fn_BristolFE_v2(mod, matls, steps, fe_options=None, return_mats=False):
 
     res = []
     plans = {}
+    elem_op = None  # element-by-element form of K, built on first use
     try:
         for s, step in enumerate(steps):
             r = {}
@@ -110,8 +112,12 @@ def fn_BristolFE_v2(mod, matls, steps, fe_options=None, return_mats=False):
             dt = t[1] - t[0]
             key = (float(dt), fe_options["solver_mode"], fe_options["solver_precision"])
             if key not in plans:
+                operator = fe_options.get("stiffness_operator", "auto")
+                if operator != "csr" and elem_op is None:
+                    elem_op = element_operator(mod, matls, fe_options, gl_lookup, K.shape[0])
                 plans[key] = make_plan(K, C, M, dt, fe_options["solver_mode"],
-                                       fe_options["solver_precision"])
+                                       fe_options["solver_precision"], element_op=elem_op,
+                                       operator=operator)
             mon_dsps, fld, _, fld_time = fn_explicit_dynamic_solver_v6(
                 K, C, M, t, frc_gi, frcs, None, None, hist_gi,
                 fe_options["field_output_every_n_frames"], fe_options["use_gpu_if_available"],

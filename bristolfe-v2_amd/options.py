@@ -23,6 +23,9 @@ BRISTOLFE_DEFAULTS = {
     "dynamic_solver_version": "v6",
     "solver_mode": "vel at curent time step",  # spelling as in the reference
     "field_output_type": "KE",
+    # not in the reference: K u by the element-by-element operator when the
+    # mesh has the template form ("auto"), always CSR ("csr"), or required ("ebe")
+    "stiffness_operator": "auto",
 }
 
 SOLVER_MODES = {
