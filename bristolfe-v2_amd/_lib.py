@@ -24,6 +24,7 @@ OPERATOR_CSR = 0
 OPERATOR_EBE = 1
 TUNE_EBE_GROUP = 4
 TUNE_EBE_KERNEL = 5
+TUNE_MULTI_ROWS_PER_WAVE = 6
 
 # Every symbol the header declares.
 EXPORTS = (
@@ -221,9 +222,9 @@ class Plan:
             frcs = frcs[None, :]
         rows = frcs.shape[0] if p > 0 else 0
         Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
-        hist = np.zeros((q, n_t), order="F")
+        hist = np.empty((q, n_t), order="F")  # the library writes every entry
         n_frames = (n_t - 1) // field_every + 1 if field_every and field_every > 0 else 0
-        fld = np.zeros((self.n, n_frames), order="F") if n_frames else None
+        fld = np.empty((self.n, n_frames), order="F") if n_frames else None
         ti = C.c_int32()
         check(self._lib.bfe_run(
             self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
@@ -265,7 +266,7 @@ class Plan:
             frcs = frcs[None, :]
         rows = frcs.shape[0] if p > 0 else 0
         Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
-        hist = np.zeros((q, n_t), order="F")
+        hist = np.empty((q, n_t), order="F")  # the library writes every entry
         ti = C.c_int32()
         check(self._lib.bfe_run_checkpointed(
             self._h, int(slot), int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p),
@@ -292,7 +293,7 @@ class Plan:
             frcs = frcs[None, :]
         rows = frcs.shape[0] if p > 0 else 0
         Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
-        hist = np.zeros((q, n_src, n_t), order="F")
+        hist = np.empty((q, n_src, n_t), order="F")  # the library writes every entry
         ti = C.c_int32()
         check(self._lib.bfe_run_batched(
             self._h, int(n_t), int(n_src), ptr(frc_idx, C.c_int32), ptr(frc_src, C.c_int32), p,

@@ -25,6 +25,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/bfe.h"
@@ -187,66 +188,91 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // and the u1 gathers read S contiguous values. A workgroup holds blockDim/64
 // rows (tunable); the per-row lists (forcing, monitors, interface) are indexed
 // by row. remap = 1 applies the XCD-aware workgroup order.
-template <typename T, bool DAMPED>
-__global__ __launch_bounds__(512) void k_step_multi(
-    int64_t n, int32_t S, int64_t nblk, int32_t remap, const int64_t* __restrict__ off,
+// R consecutive rows of one SELL-64 slice per wavefront (R divides 64, so they
+// share the slice width): their entry loads, shuffles and u1 gathers are
+// interleaved, R independent chains in flight per lane. The forcing,
+// interface and monitor lists are grouped per workgroup (lb).
+template <typename T, bool DAMPED, int R>
+__device__ __forceinline__ void multi_rows(
+    int64_t lb, int64_t row0, int64_t n, int32_t S, int lane, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
-    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
-    BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
-    int32_t t, int32_t q, T* __restrict__ hist_prev, IfaceRows<T> I) {
-  const int64_t lb = remap ? xcd_block(blockIdx.x, nblk) : static_cast<int64_t>(blockIdx.x);
-  const int64_t row = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;  // row is uniform across the wavefront
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, const IfaceRows<T>& I) {
   const bool active = lane < S;
-  const int64_t sl = row >> 6;
-  const int64_t b0 = off[sl] + (row & 63);
-  const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries of this row
-  // The row's entries are loaded once, one per lane, and broadcast with
+  const int64_t sl = row0 >> 6;
+  const int64_t b0 = off[sl] + (row0 & 63);
+  const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries per row of the slice
+  // The rows' entries are loaded once, one per lane, and broadcast with
   // cross-lane shuffles: the gathers of u1 then issue back to back.
-  T acc = T(0);
+  T acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = T(0);
   for (int k0 = 0; k0 < w; k0 += 64) {
     const int cnt = min(64, w - k0);
-    T myv = T(0);
-    int myc = 0;
-    if (lane < cnt) {
-      myv = val[b0 + 64 * static_cast<int64_t>(k0 + lane)];
-      myc = col[b0 + 64 * static_cast<int64_t>(k0 + lane)];
+    T myv[R];
+    int myc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      myv[r] = T(0);
+      myc[r] = 0;
+      if (lane < cnt) {
+        myv[r] = val[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
+        myc[r] = col[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
+      }
     }
     int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-      const int c0 = __shfl(myc, k), c1 = __shfl(myc, k + 1);
-      const int c2 = __shfl(myc, k + 2), c3 = __shfl(myc, k + 3);
-      const T v0 = __shfl(myv, k), v1 = __shfl(myv, k + 1);
-      const T v2 = __shfl(myv, k + 2), v3 = __shfl(myv, k + 3);
-      T x0 = T(0), x1 = T(0), x2 = T(0), x3 = T(0);
-      if (active) {
-        x0 = u1[static_cast<int64_t>(c0) * S + lane];
-        x1 = u1[static_cast<int64_t>(c1) * S + lane];
-        x2 = u1[static_cast<int64_t>(c2) * S + lane];
-        x3 = u1[static_cast<int64_t>(c3) * S + lane];
+    for (; k + 2 <= cnt; k += 2) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int c0 = __shfl(myc[r], k), c1 = __shfl(myc[r], k + 1);
+        const T v0 = __shfl(myv[r], k), v1 = __shfl(myv[r], k + 1);
+        T x0 = T(0), x1 = T(0);
+        if (active) {
+          x0 = u1[static_cast<int64_t>(c0) * S + lane];
+          x1 = u1[static_cast<int64_t>(c1) * S + lane];
+        }
+        acc[r] += v0 * x0 + v1 * x1;
       }
-      acc += v0 * x0 + v1 * x1 + v2 * x2 + v3 * x3;
     }
     for (; k < cnt; ++k) {
-      const int c0 = __shfl(myc, k);
-      const T v0 = __shfl(myv, k);
-      if (active) acc += v0 * u1[static_cast<int64_t>(c0) * S + lane];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int c0 = __shfl(myc[r], k);
+        const T v0 = __shfl(myv[r], k);
+        if (active) acc[r] += v0 * u1[static_cast<int64_t>(c0) * S + lane];
+      }
     }
   }
   if (!active) return;
-  const int64_t me = row * S + lane;
-  const T a = u1[me], b = u2[me];
-  T un;
-  if constexpr (DAMPED) {
-    const T be = beta[row];
-    un = be * a + (T(1) - be) * b - acc;
-  } else {
-    un = (T(2) * a - b) - acc;
+  const int nr = static_cast<int>(min<int64_t>(R, n - row0));
+  T a[R], b[R], un[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int64_t row = r < nr ? row0 + r : row0;  // rows past n recompute row0, then drop it
+    const int64_t me = row * S + lane;
+    a[r] = u1[me];
+    b[r] = u2[me];
+    if constexpr (DAMPED) {
+      const T be = beta[row];
+      un[r] = be * a[r] + (T(1) - be) * b[r] - acc[r];
+    } else {
+      un[r] = (T(2) * a[r] - b[r]) - acc[r];
+    }
   }
-  for (int32_t i = L.fptr[row], ie = L.fptr[row + 1]; i < ie; ++i)
-    if (L.f_src[i] == lane) un += f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
-  for (int32_t i = L.iptr[row], ie = L.iptr[row + 1]; i < ie; ++i) {
+  // forcing, interface and monitor entries of the workgroup (usually none)
+  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.f_row[i] - row0;
+    if (rr >= 0 && rr < nr && L.f_src[i] == lane) {
+      const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (rr == r) un[r] += f;
+    }
+  }
+  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.i_row[i] - row0;
+    if (rr < 0 || rr >= nr) continue;
     const int32_t k = L.i_k[i];
     T c1 = T(0), c2 = T(0);
     for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
@@ -254,13 +280,40 @@ __global__ __launch_bounds__(512) void k_step_multi(
       c1 += I.cval[p] * u1[c];
       c2 += I.cval[p] * u2[c];
     }
-    un += I.gfix[k] * b + I.sq * c1 + I.sr * c2;
-    if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un;
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (rr == r) {
+        un[r] += I.gfix[k] * b[r] + I.sq * c1 + I.sr * c2;
+        if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un[r];
+      }
   }
-  u[me] = un;
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (r < nr) u[(row0 + r) * S + lane] = un[r];
   if (hist_prev != nullptr)
-    for (int32_t i = L.mptr[row], ie = L.mptr[row + 1]; i < ie; ++i)
-      hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a;
+    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i) {
+      const int64_t rr = L.m_row[i] - row0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (rr == r && r < nr) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a[r];
+    }
+}
+
+// Wavefront v of the grid marches rows v*R .. v*R + R - 1.
+template <typename T, bool DAMPED, int R>
+__global__ __launch_bounds__(512) void k_step_multi(
+    int64_t n, int32_t S, int64_t nblk, int32_t remap, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, IfaceRows<T> I) {
+  const int64_t lb = remap ? xcd_block(blockIdx.x, nblk) : static_cast<int64_t>(blockIdx.x);
+  const int64_t wave = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t row0 = wave * R;
+  if (row0 >= n) return;  // uniform across the wavefront
+  multi_rows<T, DAMPED, R>(lb, row0, n, S, lane, off, col, val, beta, u1, u2, u, L, f_alpha, F,
+                           F_rows, t, q, hist_prev, I);
 }
 
 // ---------------------------------------------------------------------------
@@ -815,11 +868,12 @@ struct bfe_plan_s {
   std::vector<double> alpha;
   DevBuf off, col, val, beta;
   DevBuf iface, cptr, ccol, cval, gfix, xptr, xcol, xval;
-  // interface-row lists grouped per workgroup (single- and multi-source)
-  DevBuf is_ptr, is_row, is_k, im_ptr, im_row, im_k;
+  // interface-row lists grouped per workgroup of the single-source march
+  DevBuf is_ptr, is_row, is_k;
   bfe_run_stats stats{};
   int32_t timing_every = 0;
-  int32_t multi_rows = 4;   // rows (wavefronts) per workgroup of k_step_multi
+  int32_t multi_rows = 4;   // wavefronts per workgroup of k_step_multi
+  int32_t multi_rpw = 1;    // rows per wavefront of k_step_multi
   int32_t xcd_remap = 1;    // XCD-aware workgroup order
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
   // adjoint support (defined further down)
@@ -840,7 +894,7 @@ struct bfe_plan_s {
     grad_elems.reset();
     ebe.reset();
     for (DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr, &xcol,
-                      &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
+                      &xval, &is_ptr, &is_row, &is_k})
       b->release();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -850,7 +904,7 @@ struct bfe_plan_s {
   int64_t device_bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr,
-                            &xcol, &xval, &is_ptr, &is_row, &is_k, &im_ptr, &im_row, &im_k})
+                            &xcol, &xval, &is_ptr, &is_row, &is_k})
       s += static_cast<int64_t>(b->bytes);
     if (ebe) s += ebe->bytes();
     return s;
@@ -930,13 +984,10 @@ void upload_plan(bfe_plan_s& H, const bfe::Plan& P) {
   if (P.damped) H.beta = upload(cast_vec<T>(P.beta));
   const int64_t n = P.n;
   const int64_t nb_s = (n + kRowsSingle - 1) / kRowsSingle;
-  const int64_t nb_m = n;  // multi-source lists are per row
   std::vector<int32_t> ks(P.iface.size());
   for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
   Grouped gs = group_by_block(nb_s, kRowsSingle, P.iface, &ks, nullptr);
-  Grouped gm = group_by_block(nb_m, 1, P.iface, &ks, nullptr);
   H.is_ptr = upload(gs.ptr); H.is_row = upload(gs.row); H.is_k = upload(gs.a);
-  H.im_ptr = upload(gm.ptr); H.im_row = upload(gm.row); H.im_k = upload(gm.a);
   if (!P.iface.empty()) {
     H.iface = upload(P.iface);
     H.cptr = upload(P.coff.ptr);
@@ -971,14 +1022,14 @@ void check_indices(const int32_t* idx, int64_t count, int64_t n, const char* wha
 // Everything a run needs on the device besides the plan.
 struct RunBuffers {
   DevBuf F, fptr, frow, fsrc, ffrow, falpha, mptr, mrow, mk, hidx, wI;
-  BlockLists lists(const bfe_plan_s& H, bool multi) const {
+  BlockLists lists(const bfe_plan_s& H) const {
     BlockLists L;
     L.fptr = fptr.as<int32_t>(); L.f_row = frow.as<int32_t>(); L.f_src = fsrc.as<int32_t>();
     L.f_frow = ffrow.as<int32_t>();
     L.mptr = mptr.as<int32_t>(); L.m_row = mrow.as<int32_t>(); L.m_k = mk.as<int32_t>();
-    L.iptr = (multi ? H.im_ptr : H.is_ptr).as<int32_t>();
-    L.i_row = (multi ? H.im_row : H.is_row).as<int32_t>();
-    L.i_k = (multi ? H.im_k : H.is_k).as<int32_t>();
+    L.iptr = H.is_ptr.as<int32_t>();
+    L.i_row = H.is_row.as<int32_t>();
+    L.i_k = H.is_k.as<int32_t>();
     return L;
   }
 };
@@ -1071,15 +1122,19 @@ void finish_timing(bfe_plan_s& H, KernelSampler& sampler, int64_t steps, int64_t
   sampler.collect();
 }
 
+// Device -> caller's host buffer (FP64), returning the number of non-finite
+// values. FP64 results are copied straight into the caller's buffer.
 template <typename T>
 int64_t copy_out(const DevBuf& d, double* out, size_t count) {
-  std::vector<T> tmp(count);
-  BFE_HIP(hipMemcpy(tmp.data(), d.p, count * sizeof(T), hipMemcpyDeviceToHost));
-  int64_t bad = 0;
-  for (size_t i = 0; i < count; ++i) {
-    out[i] = static_cast<double>(tmp[i]);
-    bad += !std::isfinite(out[i]);
+  if constexpr (std::is_same_v<T, double>) {
+    BFE_HIP(hipMemcpy(out, d.p, count * sizeof(double), hipMemcpyDeviceToHost));
+  } else {
+    std::unique_ptr<T[]> tmp(new T[count]);
+    BFE_HIP(hipMemcpy(tmp.get(), d.p, count * sizeof(T), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < count; ++i) out[i] = static_cast<double>(tmp[i]);
   }
+  int64_t bad = 0;
+  for (size_t i = 0; i < count; ++i) bad += !std::isfinite(out[i]);
   return bad;
 }
 
@@ -1301,8 +1356,7 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   const int32_t n_frames = field_every > 0 ? (n_t - 1) / field_every + 1 : 0;
   if (n_frames > 0 && field_out == nullptr) throw std::invalid_argument("field output is NULL");
-  if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * n_t);
-  if (n_frames > 0) std::memset(field_out, 0, sizeof(double) * static_cast<size_t>(n) * n_frames);
+
   H.stats = bfe_run_stats{};
   H.stats.columns = 1;
   const int64_t t0 = p > 0 ? first_forced_step(frcs, frc_rows, n_t) : -1;
@@ -1319,7 +1373,12 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     if (p > 0) rec->F.assign(frcs, frcs + static_cast<size_t>(frc_rows) * n_t);
     rec->valid = true;
   }
-  if (t0 < 0) return;  // empty loop (v6:163-180): outputs stay zero
+  if (t0 < 0) {  // empty loop (v6:163-180): outputs are zero
+    if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * n_t);
+    if (n_frames > 0)
+      std::memset(field_out, 0, sizeof(double) * static_cast<size_t>(n) * n_frames);
+    return;
+  }
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
@@ -1331,7 +1390,7 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   RunBuffers R;
   make_run_buffers<T>(R, H, rows_per_block, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
                       hist_idx, q);
-  BlockLists L = R.lists(H, false);
+  BlockLists L = R.lists(H);
   if (E) {
     L.iptr = E->iptr.as<int32_t>();
     L.i_row = E->irow.as<int32_t>();
@@ -1494,7 +1553,7 @@ void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
   RunBuffers R;
   make_run_buffers<T>(R, H, kRowsSingle, nblk, 2, fi.data(), fs.data(), ptot, Fentry.data(),
                       ptot, static_cast<int32_t>(C), nullptr, 0);
-  const BlockLists L = R.lists(H, false);
+  const BlockLists L = R.lists(H);
   T* wI = R.wI.as<T>();
   const IfaceRows<T> IR = iface_rows<T>(H, wI);
   const T* falpha = R.falpha.as<T>();
@@ -1566,6 +1625,35 @@ void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
   BFE_HIP(hipMemcpy(g_dens, dgM.p, dgM.bytes, hipMemcpyDeviceToHost));
 }
 
+template <typename T, bool D, int R>
+void launch_multi_r(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int64_t n, int32_t S,
+                    const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
+                    const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
+                    const IfaceRows<T>& IR) {
+  hipLaunchKernelGGL((k_step_multi<T, D, R>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
+                     nblk, H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+                     D ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
+}
+
+template <typename T>
+void launch_multi(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int64_t n,
+                  int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
+                  const T* falpha, const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
+                  const IfaceRows<T>& IR) {
+  const int key = rpw * 2 + (H.damped ? 1 : 0);
+  switch (key) {
+    case 2: launch_multi_r<T, false, 1>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 3: launch_multi_r<T, true, 1>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 4: launch_multi_r<T, false, 2>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 5: launch_multi_r<T, true, 2>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 8: launch_multi_r<T, false, 4>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 9: launch_multi_r<T, true, 4>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 16: launch_multi_r<T, false, 8>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 17: launch_multi_r<T, true, 8>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    default: throw std::invalid_argument("rows per wavefront must be 1, 2, 4 or 8");
+  }
+}
+
 template <typename T>
 void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_idx,
                  const int32_t* frc_src, int32_t p, const double* frcs, int32_t frc_rows,
@@ -1576,20 +1664,33 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   if (n_src < 1 || n_src > 64) throw std::invalid_argument("n_src must be in [1, 64]");
   check_indices(frc_src, p, n_src, "forcing source");
   const int32_t S = n_src;
-  if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * S * n_t);
   H.stats = bfe_run_stats{};
   H.stats.columns = S;
   const int64_t t0 = p > 0 ? first_forced_step(frcs, frc_rows, n_t) : -1;
   if (ti_start_out) *ti_start_out = t0 < 0 ? n_t + 1 : static_cast<int32_t>(t0 + 1);
-  if (t0 < 0) return;
+  if (t0 < 0) {
+    if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * S * n_t);
+    return;
+  }
 
   BFE_HIP(hipSetDevice(H.device));
   hipStream_t st = H.stream;
   const int32_t rpb = H.multi_rows;
-  const int64_t nblk = (n + rpb - 1) / rpb;
+  const int32_t rpw = H.multi_rpw;
+  const int64_t rows_per_wg = static_cast<int64_t>(rpb) * rpw;
+  const int64_t nblk = (n + rows_per_wg - 1) / rows_per_wg;
   RunBuffers R;
-  make_run_buffers<T>(R, H, 1, n, S, frc_idx, frc_src, p, frcs, frc_rows, n_t, hist_idx, q);
-  const BlockLists L = R.lists(H, true);
+  make_run_buffers<T>(R, H, rows_per_wg, nblk, S, frc_idx, frc_src, p, frcs, frc_rows, n_t,
+                      hist_idx, q);
+  BlockLists L = R.lists(H);
+  // interface rows grouped per workgroup of this launch shape
+  std::vector<int32_t> ks(H.iface_host.size());
+  for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
+  Grouped gi = group_by_block(nblk, rows_per_wg, H.iface_host, &ks, nullptr);
+  DevBuf gi_ptr = upload(gi.ptr), gi_row = upload(gi.row), gi_k = upload(gi.a);
+  L.iptr = gi_ptr.as<int32_t>();
+  L.i_row = gi_row.as<int32_t>();
+  L.i_k = gi_k.as<int32_t>();
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * S * n_t);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * S * 3);
   BFE_HIP(hipMemsetAsync(ubuf.p, 0, ubuf.bytes, st));
@@ -1611,15 +1712,7 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     T* hp = (q > 0 && prev >= t0) ? hist + prev * per_t : nullptr;
     const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
-    if (H.damped)
-      hipLaunchKernelGGL((k_step_multi<T, true>), dim3(nblk), dim3(64 * rpb), 0, st, n, S, nblk,
-                         H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         H.beta.as<T>(), u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
-    else
-      hipLaunchKernelGGL((k_step_multi<T, false>), dim3(nblk), dim3(64 * rpb), 0, st, n, S, nblk,
-                         H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
-                         static_cast<const T*>(nullptr), u1, u2, u, L, falpha, F, frc_rows, ti,
-                         q, hp, IR);
+    launch_multi<T>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
     sampler.after(t);
     ++launches;
     if (H.nI > 0 && H.has_xinv) {
@@ -1948,6 +2041,11 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         if (value != 1 && value != 2 && value != 4 && value != 8)
           throw std::invalid_argument("rows per workgroup must be 1, 2, 4 or 8");
         h->multi_rows = value;
+        break;
+      case BFE_TUNE_MULTI_ROWS_PER_WAVE:
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+          throw std::invalid_argument("rows per wavefront must be 1, 2, 4 or 8");
+        h->multi_rpw = value;
         break;
       case BFE_TUNE_XCD_REMAP:
         h->xcd_remap = value ? 1 : 0;

@@ -201,13 +201,17 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
                        double* g_stiffness, double* g_density);
 
 /*
- * bfe_set_tuning — launch shape of the multi-source march (no counterpart in
- * the reference): rows per workgroup of the one-wavefront-per-row kernel
- * (1, 2, 4 or 8; default 4) and the XCD-aware workgroup order (default on).
- * Results do not depend on either; the autotuner picks them by timing.
+ * bfe_set_tuning — launch shapes and operator choice (no counterpart in the
+ * reference): wavefronts per workgroup of the one-wavefront-per-row
+ * multi-source kernel (1, 2, 4 or 8; default 4), the XCD-aware workgroup
+ * order (default on), and the keys below. Results do not depend on any of
+ * them beyond rounding; the autotuner picks the multi-source shape by timing.
  */
 #define BFE_TUNE_MULTI_ROWS_PER_BLOCK 1
 #define BFE_TUNE_XCD_REMAP 2
+/* rows each wavefront of the multi-source march handles together (1, 2, 4
+ * or 8; default 1): consecutive rows of one 64-row slice, interleaved */
+#define BFE_TUNE_MULTI_ROWS_PER_WAVE 6
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3

@@ -185,7 +185,8 @@ def test_plate_workload_full_size_parity():
 
 def test_batched_tuning_does_not_change_results():
     """Every launch shape of the multi-source kernel gives bit-identical
-    histories (rows per workgroup, dispatch order), and column tiles agree."""
+    histories (wavefronts per workgroup, rows per wavefront, dispatch order),
+    and column tiles agree."""
     print("case: batched tuning", flush=True)
     from bristolfe_v2_amd import _lib, autotune
     mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3)
@@ -201,10 +202,12 @@ def test_batched_tuning_does_not_change_results():
         ref, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
         for rows in (1, 2, 8):
             for remap in (0, 1):
-                plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, rows)
-                plan.set_tuning(_lib.TUNE_XCD_REMAP, remap)
-                h, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
-                np.testing.assert_array_equal(h, ref)
+                for rpw in (1, 2, 8):
+                    plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, rows)
+                    plan.set_tuning(_lib.TUNE_XCD_REMAP, remap)
+                    plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, rpw)
+                    h, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+                    np.testing.assert_array_equal(h, ref)
         h, _ = autotune.run_tiled(plan, n_t, S, src, np.arange(S), s, hi, 16)
         assert rel_l2(h, ref) < 1e-13
     finally:
