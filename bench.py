@@ -321,7 +321,8 @@ def main():
     if args.operator != "csr":
         t0 = time.perf_counter()
         op = ebe.element_operator(c["mod"], c["matls"], c["fe_options"], c["gl_lookup"], n)
-        log(f"element operator: {'none' if op is None else f'{op.tmpl.shape[0]} templates'} "
+        nt = 0 if op is None else sum(g.tmpl.shape[0] for g in op.groups)
+        log(f"element operator: {'none' if op is None else f'{nt} templates'} "
             f"({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     plan = make_plan(K, C, M, c["dt"], element_op=op, operator=args.operator)
@@ -362,11 +363,12 @@ def main():
         dmp = "true" if info["damped"] else "false"
         if pattern:
             pname = {1: "PatKuhn24", 2: "PatTriRect", 3: "PatTriIso"}[info["ebe_pattern"]]
-            kname = (f"k_step_ebe_pat<double,{dmp},{pname},{op.dpn}> (element-by-element, "
-                     "neighbour u in registers, + central-difference update)")
+            kname = " + ".join(f"k_step_ebe_pat<double,{dmp},{pname},{g.dpn}>" for g in op.groups)
+            kname += " (element-by-element, neighbour u in registers, + central-difference update)"
         else:
-            kname = (f"k_step_ebe<double,{dmp},{op.el_nodes.shape[1]},{op.dpn},8> "
-                     "(element-by-element gather + central-difference update)")
+            kname = " + ".join(f"k_step_ebe<double,{dmp},{g.el_nodes.shape[1]},{g.dpn},8>"
+                               for g in op.groups)
+            kname += " (element-by-element gather + central-difference update)"
     else:
         bpd = algorithmic_bytes_per_dof(info["nnz_k"], n, info["damped"])
         kname = "k_step<double,true> (SELL-64 SpMV + central-difference update)"
@@ -400,8 +402,9 @@ def main():
         "loop_gdof_steps_per_s_wall": round(n * steps_done * args.steps / wall / 1e9, 4),
     }
     if use_ebe:
-        out["config"]["operator"] = (f"element-by-element: {op.n_el} elements, "
-                                     f"{info['ebe_templates']} templates, {info['ebe_slots']} slots")
+        out["config"]["operator"] = (f"element-by-element: {op.n_el} elements in "
+                                     f"{info['ebe_groups']} group(s), {info['ebe_templates']} "
+                                     f"templates, {info['ebe_slots']} slots")
         out["roofline"]["stored_bytes_per_dof_step"] = round(
             ebe.ebe_stored_bytes_per_dof(op, n, info["ebe_slots"], info["damped"],
                                          info["ebe_classes"] if pattern else 0), 2)

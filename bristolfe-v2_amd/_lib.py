@@ -30,6 +30,7 @@ TUNE_MULTI_ROWS_PER_WAVE = 6
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
     "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
+    "bfe_set_element_groups",
     "bfe_set_kernel_timing",
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
@@ -45,7 +46,16 @@ class BfeInfo(C.Structure):
         ("device", C.c_int32), ("device_bytes", C.c_int64), ("operator_kind", C.c_int32),
         ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
         ("ebe_elements", C.c_int64), ("ebe_check", C.c_double), ("ebe_classes", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("ebe_groups", C.c_int32),
+    ]
+
+
+class BfeElementGroup(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64), ("dpn", C.c_int32), ("npe", C.c_int32), ("n_el", C.c_int64),
+        ("el_nodes", C.c_void_p), ("el_tmpl", C.c_void_p), ("el_scale", C.c_void_p),
+        ("n_tmpl", C.c_int32), ("reserved", C.c_int32), ("tmpl", C.c_void_p),
+        ("node_dof", C.c_void_p),
     ]
 
 
@@ -106,6 +116,8 @@ def load() -> C.CDLL:
                                              C.c_int32, _i32p, _i32p, _f64p, C.c_int32, _f64p,
                                              C.c_double]
     lib.bfe_set_element_operator.restype = C.c_int
+    lib.bfe_set_element_groups.argtypes = [C.c_void_p, C.c_int32, _p(BfeElementGroup), C.c_double]
+    lib.bfe_set_element_groups.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -234,7 +246,11 @@ class Plan:
         return hist, fld, int(ti.value)
 
     def set_element_operator(self, op, check_tol=0.0):
-        """Attach an element-by-element operator (an ``ElementOperator``)."""
+        """Attach an element-by-element operator (``ElementOperator`` or
+        ``ElementGroups`` of bristolfe_v2_amd.ebe)."""
+        if len(op.groups) > 1 or op.groups[0].node_dof is not None:
+            self.set_element_groups(op.groups, check_tol)
+            return
         el_nodes = np.ascontiguousarray(op.el_nodes, np.int32)
         n_el, npe = el_nodes.shape
         el_tmpl = np.ascontiguousarray(op.el_tmpl, np.int32).ravel()
@@ -244,6 +260,23 @@ class Plan:
             self._h, int(op.n_nodes), int(op.dpn), n_el, npe, ptr(el_nodes, C.c_int32),
             ptr(el_tmpl, C.c_int32), ptr(el_scale, C.c_double), int(tmpl.shape[0]),
             ptr(tmpl, C.c_double), float(check_tol)))
+
+    def set_element_groups(self, groups, check_tol=0.0):
+        keep = []
+        arr = (BfeElementGroup * len(groups))()
+        for i, g in enumerate(groups):
+            el_nodes = np.ascontiguousarray(g.el_nodes, np.int32)
+            el_tmpl = np.ascontiguousarray(g.el_tmpl, np.int32).ravel()
+            el_scale = np.ascontiguousarray(g.el_scale, np.float64).ravel()
+            tmpl = np.ascontiguousarray(g.tmpl, np.float64)
+            nd = None if g.node_dof is None else np.ascontiguousarray(g.node_dof, np.int32)
+            keep += [el_nodes, el_tmpl, el_scale, tmpl, nd]
+            arr[i] = BfeElementGroup(
+                int(g.n_nodes), int(g.dpn), int(el_nodes.shape[1]), int(el_nodes.shape[0]),
+                el_nodes.ctypes.data, el_tmpl.ctypes.data, el_scale.ctypes.data,
+                int(tmpl.shape[0]), 0, tmpl.ctypes.data, None if nd is None else nd.ctypes.data)
+        check(self._lib.bfe_set_element_groups(self._h, len(groups), arr, float(check_tol)))
+        del keep
 
     def set_gradient_elements(self, el_dofs, el_K, el_m, el_mu):
         """el_dofs (n_el, k) 0-based (-1 unused), el_K (n_el, k, k), el_m (n_el, k), el_mu (n_el)."""

@@ -404,7 +404,7 @@ __device__ __forceinline__ void ebe_group_apply(int32_t k0, const int32_t* e0, c
     for (int b = 0; b < NPE; ++b) {
 #pragma unroll
       for (int d = 0; d < DPN; ++d) {
-        const T x = u1[static_cast<int64_t>(w[b]) * DPN + d];
+        const T x = u1[static_cast<int64_t>(w[b]) + d];  // w: the vertex's first DOF
 #pragma unroll
         for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x;
       }
@@ -432,17 +432,19 @@ __device__ __forceinline__ void ebe_load_records(const int32_t* e0, const int4* 
   }
 }
 
-// The update of node v's DPN rows from acc = (K u1) on those rows, with the
-// forcing, interface, field and monitor handling of k_step. alpha and beta
-// are per DOF, or per node when coef_per_node is set.
+// The update of node v's DPN rows (global rows r0 .. r0 + DPN - 1) from
+// acc = (K u1) on those rows, with the forcing, interface, field and monitor
+// handling of k_step; the lists hold the group's local rows v*DPN + c. alpha
+// and beta are per DOF, or per node when coef_per_node is set.
 template <typename T, bool DAMPED, int DPN>
 __device__ __forceinline__ void ebe_finish(
-    int64_t v, int64_t lb, const T* acc, const T* __restrict__ alpha, const T* __restrict__ beta,
-    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
-    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
-    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
-    const IfaceRows<T>& I, int32_t coef_per_node) {
-  const int64_t r0 = v * DPN;
+    int64_t v, int64_t lb, int64_t r0, const T* acc, const T* __restrict__ alpha,
+    const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
+    T* __restrict__ u, const BlockLists& L, const T* __restrict__ f_alpha,
+    const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q, T* __restrict__ hist_prev,
+    T* __restrict__ fld_prev, int32_t fld_is_vel, T dt, const IfaceRows<T>& I,
+    int32_t coef_per_node) {
+  const int64_t l0 = v * DPN;  // the node's rows in the group's local row numbering
   T a[DPN], b[DPN], un[DPN];
 #pragma unroll
   for (int c = 0; c < DPN; ++c) {
@@ -464,7 +466,7 @@ __device__ __forceinline__ void ebe_finish(
       const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
 #pragma unroll
       for (int c = 0; c < DPN; ++c)
-        if (rr == r0 + c) un[c] += f;
+        if (rr == l0 + c) un[c] += f;
     }
   }
   for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
@@ -478,7 +480,7 @@ __device__ __forceinline__ void ebe_finish(
       }
 #pragma unroll
       for (int c = 0; c < DPN; ++c)
-        if (rr == r0 + c) {
+        if (rr == l0 + c) {
           un[c] += I.gfix[k] * b[c] + I.sq * c1 + I.sr * c2;
           if (I.wI != nullptr) I.wI[k] = un[c];
         }
@@ -495,7 +497,7 @@ __device__ __forceinline__ void ebe_finish(
       if (rr / DPN == v) {
 #pragma unroll
         for (int c = 0; c < DPN; ++c)
-          if (rr == r0 + c) hist_prev[L.m_k[i]] = a[c];
+          if (rr == l0 + c) hist_prev[L.m_k[i]] = a[c];
       }
     }
 }
@@ -523,8 +525,9 @@ __device__ __forceinline__ void ebe_gather(int64_t n_nodes, int64_t v, int32_t n
 
 template <typename T, bool DAMPED, int NPE, int DPN, int G>
 __global__ __launch_bounds__(kBlock) void k_step_ebe(
-    int64_t n_nodes, int64_t nblk, int32_t n_slot, const int32_t* __restrict__ adj,
-    const int4* __restrict__ el_nodes, const T* __restrict__ el_scale,
+    int64_t n_nodes, int64_t nblk, const int32_t* __restrict__ dof0, int32_t n_slot,
+    const int32_t* __restrict__ adj, const int4* __restrict__ el_nodes,
+    const T* __restrict__ el_scale,
     const T* __restrict__ srows, const T* __restrict__ alpha, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
@@ -537,8 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe(
 #pragma unroll
   for (int c = 0; c < DPN; ++c) acc[c] = T(0);
   ebe_gather<T, NPE, DPN, G>(n_nodes, v, n_slot, adj, el_nodes, el_scale, srows, u1, acc);
-  ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
-                             hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+  const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
+  ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
+                             q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
 }
 
 // ---------------------------------------------------------------------------
@@ -586,15 +590,17 @@ struct ClassOffsets {
 
 // Element-by-element step on a mesh pattern (at most 128 VGPRs, so that four
 // wavefronts share a SIMD: the kernel is bound by load latency more than by
-// its FP64 FMAs). The NC neighbours' u1 values are
-// gathered once and kept in registers; on a regular node neighbour j is node
-// v + off[j] (irregular nodes, flagged in bit 31 of pmask, read the class-major
+// its FP64 FMAs). Node v's first DOF is r0 (dof0[v], or v*DPN when dof0 is
+// NULL). The NC neighbours' u1 values are gathered once and kept in
+// registers; on a regular node the first DOF of neighbour j is r0 + off[j]
+// (irregular nodes, flagged in bit 31 of pmask, read it from the class-major
 // neighbour table nbr). pmask bit k marks an occupied slot, smask bit k a slot
 // whose element scale differs from 1: only those read the slot table and the
 // scale. Template rows (canonical vertex order) come through the scalar cache.
 template <typename T, bool DAMPED, typename PT, int DPN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
-    int64_t n_nodes, int64_t nblk, ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
+    int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
+    ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
     const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
     const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
@@ -608,6 +614,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   if (v >= n_nodes) return;
   const uint32_t pm = pmask[v], sm = smask[v];
   const int64_t ib = pm >> 31;  // 1: irregular node
+  const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
   constexpr uint32_t kAll = (1u << NS) - 1u;
   if (__all((pm & kAll) == kAll && sm == 0u)) {
     // every lane's elements are all present with scale 1 (the interior): K u
@@ -618,22 +625,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     for (int c = 0; c < DPN; ++c) acc[c] = T(0);
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
-      int64_t w = v;
+      int64_t w = r0;
       if (j != PT::SELF) {
         const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
-        const int64_t wr = v + co.off[j];
-        const int64_t wi = wj < 0 ? v : wj;
-        w = ib * wi + (1 - ib) * (wr < 0 || wr >= n_nodes ? v : wr);
+        const int64_t wr = r0 + co.off[j];
+        const int64_t wi = wj < 0 ? r0 : wj;
+        w = ib * wi + (1 - ib) * (wr < 0 || wr > n - DPN ? r0 : wr);
       }
 #pragma unroll
       for (int d = 0; d < DPN; ++d) {
-        const T xd = u1[w * DPN + d];
+        const T xd = u1[w + d];
 #pragma unroll
         for (int c = 0; c < DPN; ++c) acc[c] += stencil[(j * DPN + c) * DPN + d] * xd;
       }
     }
-    ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
-                               hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+    ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
+                               q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
     return;
   }
   // Loads are issued on every lane without branching (a divergent branch
@@ -642,15 +649,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   T x[NC][DPN];
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
-    int64_t w = v;
+    int64_t w = r0;
     if (j != PT::SELF) {
       const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
-      const int64_t wr = v + co.off[j];  // outside [0, n): absent, used only by empty slots
-      const int64_t wi = wj < 0 ? v : wj;
-      w = ib * wi + (1 - ib) * (wr < 0 || wr >= n_nodes ? v : wr);
+      const int64_t wr = r0 + co.off[j];  // out of range: absent, used only by empty slots
+      const int64_t wi = wj < 0 ? r0 : wj;
+      w = ib * wi + (1 - ib) * (wr < 0 || wr > n - DPN ? r0 : wr);
     }
 #pragma unroll
-    for (int d = 0; d < DPN; ++d) x[j][d] = u1[w * DPN + d];
+    for (int d = 0; d < DPN; ++d) x[j][d] = u1[w + d];
   }
   T acc[DPN];
 #pragma unroll
@@ -677,8 +684,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #pragma unroll
     for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
   }
-  ebe_finish<T, DAMPED, DPN>(v, lb, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q,
-                             hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+  ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
+                             q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
 }
 
 // Pattern id of a layout's class table, 0 if none matches.
@@ -831,21 +838,42 @@ Grouped group_by_block(int64_t nblk, int64_t rows_per_block, const std::vector<i
   return G;
 }
 
-// Element-by-element operator (k_step_ebe) attached to a handle.
-struct EbeOp {
+// One group of the element-by-element operator: elements of one type over
+// the group's own node numbering (k_step_ebe / k_step_ebe_pat).
+struct EbeGroup {
   int64_t n_nodes = 0, n_el = 0;
   int32_t dpn = 0, npe = 0, n_slot = 0, n_tmpl = 0, n_cls = 0;
-  int32_t pattern = 0;  // mesh pattern of k_step_ebe_pat, 0: none
-  int32_t coef_per_node = 0;  // alpha (and beta) stored per node
-  std::vector<int64_t> cls_off;
-  DevBuf adj, nodes, scale, srows, alpha, beta, nbr, srows_c, pmask, smask, stencil;
-  DevBuf iptr, irow, ik;  // interface rows grouped per k_step_ebe workgroup
-  double check_err = 0.0;  // largest relative difference to the CSR step at set-up
+  int32_t pattern = 0;       // mesh pattern of k_step_ebe_pat, 0: none
+  bool uniform = false;      // node j's first DOF is j*dpn (no DOF table read)
+  std::vector<int64_t> doff; // first-DOF offset of each neighbour class on regular nodes
+  DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil;
+  DevBuf iptr, irow, ik;     // interface rows (group-local rows) per workgroup
   int64_t bytes() const {
     int64_t s = 0;
-    for (const DevBuf* b : {&adj, &nodes, &scale, &srows, &alpha, &beta, &nbr, &srows_c, &pmask,
-                            &smask, &stencil, &iptr, &irow, &ik})
+    for (const DevBuf* b : {&dof0, &adj, &nodes, &scale, &srows, &nbr, &srows_c, &pmask, &smask,
+                            &stencil, &iptr, &irow, &ik})
       s += static_cast<int64_t>(b->bytes);
+    return s;
+  }
+};
+
+// Element-by-element operator attached to a handle: K = sum over groups.
+struct EbeOp {
+  std::vector<EbeGroup> groups;
+  // global row -> (group, group-local row); group -1 when no group owns it
+  std::vector<int8_t> row_group;
+  std::vector<int32_t> row_local;
+  int32_t coef_per_node = 0;  // one group, uniform numbering, alpha/beta equal per node
+  DevBuf alpha, beta;         // per DOF, or per node (coef_per_node)
+  double check_err = 0.0;     // largest relative difference to the CSR step at set-up
+  int64_t n_el() const {
+    int64_t s = 0;
+    for (const auto& g : groups) s += g.n_el;
+    return s;
+  }
+  int64_t bytes() const {
+    int64_t s = static_cast<int64_t>(alpha.bytes + beta.bytes);
+    for (const auto& g : groups) s += g.bytes();
     return s;
   }
 };
@@ -1202,113 +1230,175 @@ void launch_step(const bfe_plan_s& H, int S, int64_t nblk, const T* u1, const T*
 }
 
 template <typename T, bool D, int NPE, int DPN, int G>
-void launch_ebe_group(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
-                      T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
-                      int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
-                      const IfaceRows<T>& IR) {
+void launch_ebe_generic_g(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int64_t nblk,
+                          const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
+                          const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp,
+                          int32_t fld_is_vel, T dt, const IfaceRows<T>& IR) {
   hipLaunchKernelGGL((k_step_ebe<T, D, NPE, DPN, G>), dim3(nblk), dim3(kBlock), 0, H.stream,
-                     E.n_nodes, nblk, E.n_slot, E.adj.as<int32_t>(), E.nodes.as<int4>(),
-                     E.scale.as<T>(), E.srows.as<T>(), E.alpha.as<T>(),
-                     H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q,
-                     hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
+                     Gr.n_nodes, nblk, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), Gr.n_slot,
+                     Gr.adj.as<int32_t>(), Gr.nodes.as<int4>(), Gr.scale.as<T>(),
+                     Gr.srows.as<T>(), E.alpha.as<T>(), D ? E.beta.as<T>() : nullptr, u1, u2, u,
+                     L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
 template <typename T, bool D, int NPE, int DPN>
-void launch_ebe_kernel(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
-                       T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
-                       int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
-                       const IfaceRows<T>& IR) {
+void launch_ebe_generic(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int64_t nblk,
+                        const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
+                        const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp,
+                        int32_t fld_is_vel, T dt, const IfaceRows<T>& IR) {
   switch (H.ebe_group) {
-    case 2: launch_ebe_group<T, D, NPE, DPN, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case 8: launch_ebe_group<T, D, NPE, DPN, 8>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    default: launch_ebe_group<T, D, NPE, DPN, 4>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-  }
-}
-
-template <typename T, bool D>
-void launch_ebe_damped(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1,
-                       const T* u2, T* u, const BlockLists& L, const T* falpha, const T* F,
-                       int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel,
-                       T dt, const IfaceRows<T>& IR) {
-  const int key = E.npe * 10 + E.dpn;
-  switch (key) {
-    case 31: launch_ebe_kernel<T, D, 3, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case 32: launch_ebe_kernel<T, D, 3, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case 33: launch_ebe_kernel<T, D, 3, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case 43: launch_ebe_kernel<T, D, 4, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    default: throw std::invalid_argument("unsupported element operator shape");
+    case 2: launch_ebe_generic_g<T, D, NPE, DPN, 2>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    case 4: launch_ebe_generic_g<T, D, NPE, DPN, 4>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
+    default: launch_ebe_generic_g<T, D, NPE, DPN, 8>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
   }
 }
 
 template <typename T, bool D, typename PT, int DPN>
-void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
-                    T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
-                    int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
-                    const IfaceRows<T>& IR) {
+void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int64_t nblk,
+                    const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
+                    const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp,
+                    int32_t fld_is_vel, T dt, const IfaceRows<T>& IR) {
   ClassOffsets<PT::NC> co;
-  for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(E.cls_off[j]);
+  for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(Gr.doff[j]);
   hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN>), dim3(nblk), dim3(kBlock), 0, H.stream,
-                     E.n_nodes, nblk, co, E.pmask.as<uint32_t>(), E.smask.as<uint32_t>(),
-                     E.nbr.as<int32_t>(), E.adj.as<int32_t>(), E.scale.as<T>(), E.srows_c.as<T>(),
-                     E.stencil.as<T>(), E.alpha.as<T>(), H.damped ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha,
-                     F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
+                     Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), co,
+                     Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
+                     Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.srows_c.as<T>(),
+                     Gr.stencil.as<T>(), E.alpha.as<T>(), D ? E.beta.as<T>() : nullptr, u1, u2,
+                     u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
+// Whether a kernel exists for the group's shape (generic) or pattern.
+bool has_generic_kernel(int npe, int dpn) {
+  return (npe == 3 && dpn >= 1 && dpn <= 3) || (npe == 4 && dpn == 3);
+}
+bool has_pattern_kernel(int pattern, int dpn) {
+  if (pattern == PatKuhn24::ID) return dpn == 3;
+  return (pattern == PatTriRect::ID || pattern == PatTriIso::ID) && dpn >= 1 && dpn <= 3;
+}
+
+// One group's step kernel (pattern kernel unless generic or switched off).
 template <typename T, bool D>
-void launch_ebe_pattern(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1,
-                        const T* u2, T* u, const BlockLists& L, const T* falpha, const T* F,
-                        int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel,
-                        T dt, const IfaceRows<T>& IR) {
-  const int key = E.pattern * 10 + E.dpn;
-  switch (key) {
-    case PatKuhn24::ID * 10 + 3: launch_ebe_pat<T, D, PatKuhn24, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriRect::ID * 10 + 1: launch_ebe_pat<T, D, PatTriRect, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriRect::ID * 10 + 2: launch_ebe_pat<T, D, PatTriRect, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriRect::ID * 10 + 3: launch_ebe_pat<T, D, PatTriRect, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriIso::ID * 10 + 1: launch_ebe_pat<T, D, PatTriIso, 1>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriIso::ID * 10 + 2: launch_ebe_pat<T, D, PatTriIso, 2>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    case PatTriIso::ID * 10 + 3: launch_ebe_pat<T, D, PatTriIso, 3>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); break;
-    default: throw std::invalid_argument("no pattern kernel for this element operator");
+void launch_ebe_one(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int64_t nblk,
+                    const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
+                    const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp,
+                    int32_t fld_is_vel, T dt, const IfaceRows<T>& IR, bool generic) {
+  if (!generic && H.ebe_kernel && Gr.pattern != 0) {
+    switch (Gr.pattern * 10 + Gr.dpn) {
+      case PatKuhn24::ID * 10 + 3: launch_ebe_pat<T, D, PatKuhn24, 3>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriRect::ID * 10 + 1: launch_ebe_pat<T, D, PatTriRect, 1>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriRect::ID * 10 + 2: launch_ebe_pat<T, D, PatTriRect, 2>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriRect::ID * 10 + 3: launch_ebe_pat<T, D, PatTriRect, 3>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriIso::ID * 10 + 1: launch_ebe_pat<T, D, PatTriIso, 1>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriIso::ID * 10 + 2: launch_ebe_pat<T, D, PatTriIso, 2>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      case PatTriIso::ID * 10 + 3: launch_ebe_pat<T, D, PatTriIso, 3>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+      default: throw std::invalid_argument("no pattern kernel for this element group");
+    }
+  }
+  switch (Gr.npe * 10 + Gr.dpn) {
+    case 31: launch_ebe_generic<T, D, 3, 1>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+    case 32: launch_ebe_generic<T, D, 3, 2>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+    case 33: launch_ebe_generic<T, D, 3, 3>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+    case 43: launch_ebe_generic<T, D, 4, 3>(H, E, Gr, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR); return;
+    default: throw std::invalid_argument("unsupported element group shape");
   }
 }
 
-// Whether a pattern kernel exists for the operator's (pattern, dof per node).
-bool has_pattern_kernel(const EbeOp& E) {
-  if (E.pattern == PatKuhn24::ID) return E.dpn == 3;
-  return (E.pattern == PatTriRect::ID || E.pattern == PatTriIso::ID) && E.dpn >= 1 && E.dpn <= 3;
-}
+// Per-group lists of one run (group-local rows).
+struct EbeRunLists {
+  DevBuf fptr, frow, fsrc, ffrow, falpha, mptr, mrow, mk;
+};
 
+// Grid of one group: one thread per node.
+inline int64_t ebe_blocks(const EbeGroup& Gr) { return (Gr.n_nodes + kBlock - 1) / kBlock; }
+
+// Builds the forcing and monitor lists of every group for one run (the same
+// last-wins deduplication as make_run_buffers).
 template <typename T>
-void launch_step_ebe(const bfe_plan_s& H, const EbeOp& E, int64_t nblk, const T* u1, const T* u2,
-                     T* u, const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
-                     int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
-                     const IfaceRows<T>& IR, bool generic = false) {
-  if (!generic && H.ebe_kernel && E.pattern != 0) {
-    if (H.damped)
-      launch_ebe_pattern<T, true>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
-                                  fld_is_vel, dt, IR);
-    else
-      launch_ebe_pattern<T, false>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
-                                   fld_is_vel, dt, IR);
-    return;
+std::vector<EbeRunLists> ebe_run_lists(const bfe_plan_s& H, const EbeOp& E, const int32_t* frc_idx,
+                                       int32_t p, int32_t frc_rows, const int32_t* hist_idx,
+                                       int32_t q) {
+  std::vector<std::pair<int32_t, int32_t>> key;  // (dof, entry)
+  for (int32_t i = 0; i < p; ++i) key.push_back({frc_idx[i], i});
+  std::stable_sort(key.begin(), key.end(), [](auto& a, auto& b) { return a.first < b.first; });
+  const size_t ng = E.groups.size();
+  std::vector<std::vector<int32_t>> frows(ng), fsrc(ng), ffrow(ng), mrows(ng), mks(ng);
+  std::vector<std::vector<double>> falpha(ng);
+  for (size_t i = 0; i < key.size(); ++i)
+    if (i + 1 == key.size() || key[i + 1].first != key[i].first) {
+      const int32_t r = key[i].first;
+      const int g = E.row_group[r];
+      if (g < 0) throw std::invalid_argument("forcing on a DOF that no element group owns");
+      frows[g].push_back(E.row_local[r]);
+      fsrc[g].push_back(0);
+      ffrow[g].push_back(frc_rows == 1 ? 0 : key[i].second);
+      falpha[g].push_back(H.alpha[r]);
+    }
+  for (int32_t k = 0; k < q; ++k) {
+    const int32_t r = hist_idx[k];
+    const int g = E.row_group[r];
+    if (g < 0) throw std::invalid_argument("monitor on a DOF that no element group owns");
+    mrows[g].push_back(E.row_local[r]);
+    mks[g].push_back(k);
   }
-  if (H.damped)
-    launch_ebe_damped<T, true>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
-                               fld_is_vel, dt, IR);
-  else
-    launch_ebe_damped<T, false>(H, E, nblk, u1, u2, u, L, falpha, F, F_rows, t, q, hp, fp,
-                                fld_is_vel, dt, IR);
+  std::vector<EbeRunLists> out(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    const EbeGroup& Gr = E.groups[g];
+    const int64_t nb = ebe_blocks(Gr), rpb = static_cast<int64_t>(kBlock) * Gr.dpn;
+    Grouped gf = group_by_block(nb, rpb, frows[g], &fsrc[g], &ffrow[g]);
+    // falpha follows the grouped order: regroup it with the rows as payload index
+    std::vector<int32_t> idx(frows[g].size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = static_cast<int32_t>(i);
+    Grouped gfi = group_by_block(nb, rpb, frows[g], &idx, nullptr);
+    std::vector<double> fa(gfi.a.size());
+    for (size_t i = 0; i < fa.size(); ++i) fa[i] = falpha[g][gfi.a[i]];
+    Grouped gm = group_by_block(nb, rpb, mrows[g], &mks[g], nullptr);
+    EbeRunLists& R = out[g];
+    R.fptr = upload(gf.ptr); R.frow = upload(gf.row); R.fsrc = upload(gf.a);
+    R.ffrow = upload(gf.b); R.falpha = upload(cast_vec<T>(fa));
+    R.mptr = upload(gm.ptr); R.mrow = upload(gm.row); R.mk = upload(gm.a);
+  }
+  return out;
+}
+
+inline BlockLists ebe_lists(const EbeRunLists& R, const EbeGroup& Gr) {
+  BlockLists L;
+  L.fptr = R.fptr.as<int32_t>(); L.f_row = R.frow.as<int32_t>(); L.f_src = R.fsrc.as<int32_t>();
+  L.f_frow = R.ffrow.as<int32_t>();
+  L.mptr = R.mptr.as<int32_t>(); L.m_row = R.mrow.as<int32_t>(); L.m_k = R.mk.as<int32_t>();
+  L.iptr = Gr.iptr.as<int32_t>(); L.i_row = Gr.irow.as<int32_t>(); L.i_k = Gr.ik.as<int32_t>();
+  return L;
+}
+
+// One step of every group (each group writes only its own rows).
+template <typename T>
+void launch_step_ebe(const bfe_plan_s& H, const EbeOp& E, const std::vector<EbeRunLists>* RL,
+                     const BlockLists* L0, const T* u1, const T* u2, T* u, const T* F,
+                     int32_t F_rows, int32_t t, int32_t q, T* hp, T* fp, int32_t fld_is_vel, T dt,
+                     const IfaceRows<T>& IR, bool generic = false) {
+  for (size_t g = 0; g < E.groups.size(); ++g) {
+    const EbeGroup& Gr = E.groups[g];
+    const BlockLists L = RL ? ebe_lists((*RL)[g], Gr) : *L0;
+    const T* falpha = RL ? (*RL)[g].falpha.as<T>() : nullptr;
+    if (H.damped)
+      launch_ebe_one<T, true>(H, E, Gr, ebe_blocks(Gr), u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                              fp, fld_is_vel, dt, IR, generic);
+    else
+      launch_ebe_one<T, false>(H, E, Gr, ebe_blocks(Gr), u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                               fp, fld_is_vel, dt, IR, generic);
+  }
 }
 
 // One step of both operators from the same random state (u2 = 0, no
 // forcing, monitors or interface correction): returns
-// max |u_ebe - u_csr| / max |u_csr|.
+// max |u_ebe - u_csr| / max |u_csr| over the rows the groups own.
 template <typename T>
 double compare_ebe_with_csr(const bfe_plan_s& H, const EbeOp& E, bool generic) {
   const int64_t n = H.n;
   const int64_t nb_c = (n + kRowsSingle - 1) / kRowsSingle;
-  const int64_t nb_e = (E.n_nodes + kBlock - 1) / kBlock;
-  std::vector<int32_t> zeros(std::max(nb_c, nb_e) + 1, 0);
+  int64_t nb_max = nb_c;
+  for (const auto& g : E.groups) nb_max = std::max(nb_max, ebe_blocks(g));
+  std::vector<int32_t> zeros(nb_max + 1, 0);
   DevBuf z = upload(zeros);
   BlockLists L0;
   L0.fptr = L0.f_row = L0.f_src = L0.f_frow = z.as<int32_t>();
@@ -1331,8 +1421,8 @@ double compare_ebe_with_csr(const bfe_plan_s& H, const EbeOp& E, bool generic) {
   const T dt = static_cast<T>(H.dt);
   launch_step<T>(H, 1, nb_c, dx.as<T>(), u2, y1, L0, nullptr, nullptr, 1, 0, 0, nullptr, nullptr,
                  0, dt, I, nullptr, nullptr);
-  launch_step_ebe<T>(H, E, nb_e, dx.as<T>(), u2, y2, L0, nullptr, nullptr, 1, 0, 0, nullptr,
-                     nullptr, 0, dt, I, generic);
+  launch_step_ebe<T>(H, E, nullptr, &L0, dx.as<T>(), u2, y2, nullptr, 1, 0, 0, nullptr, nullptr,
+                     0, dt, I, generic);
   BFE_HIP(hipGetLastError());
   BFE_HIP(hipStreamSynchronize(H.stream));
   std::vector<T> a(n), b(n);
@@ -1385,17 +1475,13 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   // the checkpointed forward stays on the CSR operator that the adjoint's
   // recomputation uses
   const EbeOp* E = (H.ebe && H.use_ebe && rec == nullptr) ? H.ebe.get() : nullptr;
-  const int64_t nblk = E ? (E->n_nodes + kBlock - 1) / kBlock : (n + kRowsSingle - 1) / kRowsSingle;
-  const int64_t rows_per_block = E ? static_cast<int64_t>(kBlock) * E->dpn : kRowsSingle;
+  const int64_t nblk = (n + kRowsSingle - 1) / kRowsSingle;
   RunBuffers R;
-  make_run_buffers<T>(R, H, rows_per_block, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
+  make_run_buffers<T>(R, H, kRowsSingle, nblk, 1, frc_idx, nullptr, p, frcs, frc_rows, n_t,
                       hist_idx, q);
-  BlockLists L = R.lists(H);
-  if (E) {
-    L.iptr = E->iptr.as<int32_t>();
-    L.i_row = E->irow.as<int32_t>();
-    L.i_k = E->ik.as<int32_t>();
-  }
+  const BlockLists L = R.lists(H);
+  std::vector<EbeRunLists> RL;  // per element group, group-local rows
+  if (E) RL = ebe_run_lists<T>(H, *E, frc_idx, p, frc_rows, hist_idx, q);
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * n_t);
   DevBuf dfld(sizeof(T) * static_cast<size_t>(n) * n_frames);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * 3);
@@ -1442,8 +1528,8 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     }
     sampler.before(t);
     if (E)
-      launch_step_ebe<T>(H, *E, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t),
-                         q, hp, fp, field_is_vel, dtT, IR);
+      launch_step_ebe<T>(H, *E, &RL, nullptr, u1, u2, u, F, frc_rows, static_cast<int32_t>(t), q,
+                         hp, fp, field_is_vel, dtT, IR);
     else
       launch_step<T>(H, 1, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t), q,
                      hp, fp, field_is_vel, dtT, IR, store, nullptr);
@@ -1733,6 +1819,216 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
 
 }  // namespace
 
+namespace {
+
+// Builds, checks and attaches the element-by-element operator (see
+// bfe_set_element_groups). Throws on malformed input or when the operator
+// does not reproduce the CSR step.
+void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group* groups,
+                        double check_tol) {
+  if (h == nullptr) throw std::invalid_argument("NULL handle");
+  if (n_groups < 1 || n_groups > 8) throw std::invalid_argument("1 to 8 element groups");
+  if (groups == nullptr) throw std::invalid_argument("NULL element groups");
+  const int64_t n = h->n;
+  auto E = std::make_unique<EbeOp>();
+  E->row_group.assign(n, -1);
+  E->row_local.assign(n, 0);
+  BFE_HIP(hipSetDevice(h->device));
+  for (int32_t gi = 0; gi < n_groups; ++gi) {
+    const bfe_element_group& in = groups[gi];
+    const int64_t nn = in.n_nodes, ne = in.n_el;
+    const int32_t dpn = in.dpn, npe = in.npe;
+    if (!has_generic_kernel(npe, dpn))
+      throw std::invalid_argument("element group: supported (nodes, DOF per node) are "
+                                  "(3, 1), (3, 2), (3, 3) and (4, 3)");
+    if (nn < 1 || nn > INT32_MAX || ne < 1 || ne > INT32_MAX)
+      throw std::invalid_argument("element group: node or element count out of range");
+    if (in.n_tmpl < 1) throw std::invalid_argument("element group: no templates");
+    if (!in.el_nodes || !in.el_tmpl || !in.el_scale || !in.tmpl)
+      throw std::invalid_argument("element group: NULL array");
+    if (in.node_dof == nullptr && (n_groups != 1 || nn * dpn != n))
+      throw std::invalid_argument("element group without a DOF table: n must equal "
+                                  "n_nodes * dof_per_node (node-major numbering)");
+    const int KE = npe * dpn;
+    for (int64_t e = 0; e < ne; ++e) {
+      if (in.el_tmpl[e] < 0 || in.el_tmpl[e] >= in.n_tmpl)
+        throw std::invalid_argument("element template index out of range");
+      if (!std::isfinite(in.el_scale[e])) throw std::invalid_argument("non-finite element scale");
+      for (int l = 0; l < npe; ++l) {
+        const int32_t v = in.el_nodes[e * npe + l];
+        if (v < 0 || v >= nn) throw std::invalid_argument("element node index out of range");
+      }
+    }
+    for (int64_t i = 0; i < static_cast<int64_t>(in.n_tmpl) * KE * KE; ++i)
+      if (!std::isfinite(in.tmpl[i])) throw std::invalid_argument("non-finite template entry");
+    // first DOF of every node; every (node, component) owned by one group
+    std::vector<int32_t> dof0(nn);
+    for (int64_t j = 0; j < nn; ++j) {
+      const int64_t d = in.node_dof ? in.node_dof[j] : j * dpn;
+      if (d < 0 || d + dpn > n) throw std::invalid_argument("element group DOF out of range");
+      dof0[j] = static_cast<int32_t>(d);
+      for (int c = 0; c < dpn; ++c) {
+        if (E->row_group[d + c] != -1)
+          throw std::invalid_argument("a DOF belongs to two element groups");
+        E->row_group[d + c] = static_cast<int8_t>(gi);
+        E->row_local[d + c] = static_cast<int32_t>(j * dpn + c);
+      }
+    }
+    bfe::EbeLayout Lay = bfe::build_ebe_layout(nn, dpn, ne, npe, in.el_nodes, in.el_tmpl,
+                                                in.n_tmpl, in.tmpl, kEbeSlotPad, 64);
+    EbeGroup Gr;
+    Gr.n_nodes = nn;
+    Gr.n_el = ne;
+    Gr.dpn = dpn;
+    Gr.npe = npe;
+    Gr.n_slot = Lay.n_slot_padded;
+    Gr.n_tmpl = in.n_tmpl;
+    Gr.n_cls = Lay.n_cls;
+    Gr.uniform = in.node_dof == nullptr;
+    Gr.pattern = match_pattern(Lay);
+    if (Gr.pattern != 0 && !has_pattern_kernel(Gr.pattern, dpn)) Gr.pattern = 0;
+    // element records hold each vertex's first DOF
+    std::vector<int32_t> nodes4(Lay.nodes4.size());
+    for (size_t i = 0; i < nodes4.size(); ++i) nodes4[i] = dof0[Lay.nodes4[i]];
+    Gr.adj = upload(Lay.adj);
+    Gr.nodes = upload(nodes4);
+    if (!Gr.uniform) Gr.dof0 = upload(dof0);
+    std::vector<double> stencil;
+    if (Gr.pattern != 0) {
+      const int32_t nc = Lay.n_cls;
+      // first-DOF offsets of the neighbour classes, from a node with every class present
+      Gr.doff.assign(nc, 0);
+      int64_t ref = -1;
+      for (int64_t v = 0; v < nn && ref < 0; ++v) {
+        bool all = true;
+        for (int32_t j = 0; j < nc && all; ++j) all = Lay.nbr[static_cast<size_t>(j) * nn + v] >= 0;
+        if (all) ref = v;
+      }
+      if (ref < 0) throw std::invalid_argument("element group: no interior node");
+      for (int32_t j = 0; j < nc; ++j)
+        Gr.doff[j] = static_cast<int64_t>(dof0[Lay.nbr[static_cast<size_t>(j) * nn + ref]]) -
+                     dof0[ref];
+      // neighbour table as first DOFs; irregular flag (bit 31) in DOF terms
+      std::vector<int32_t> nbrdof(Lay.nbr.size());
+      std::vector<uint32_t> pm(Lay.present);
+      for (int64_t v = 0; v < nn; ++v) {
+        bool irregular = false;
+        for (int32_t j = 0; j < nc; ++j) {
+          const int32_t w = Lay.nbr[static_cast<size_t>(j) * nn + v];
+          nbrdof[static_cast<size_t>(j) * nn + v] = w < 0 ? -1 : dof0[w];
+          if (w >= 0 && dof0[w] != dof0[v] + Gr.doff[j]) irregular = true;
+        }
+        pm[v] = (pm[v] & 0x7fffffffu) | (irregular ? 0x80000000u : 0u);
+      }
+      // slots whose element scale is not exactly 1 (absorbing layer)
+      std::vector<uint32_t> sm(nn, 0u);
+      for (int32_t k = 0; k < Lay.n_slot; ++k)
+        for (int64_t v = 0; v < nn; ++v) {
+          const int32_t e = Lay.adj[static_cast<size_t>(k) * nn + v];
+          if (e >= 0 && in.el_scale[e] != 1.0) sm[v] |= 1u << k;
+        }
+      Gr.nbr = upload(nbrdof);
+      Gr.pmask = upload(pm);
+      Gr.smask = upload(sm);
+      // summed stencil: B_j = sum over (slot, vertex) of class j
+      stencil.assign(static_cast<size_t>(nc) * dpn * dpn, 0.0);
+      for (int32_t k = 0; k < Lay.n_slot; ++k)
+        for (int b = 0; b < npe; ++b) {
+          const int32_t j = Lay.cls[k * npe + b];
+          for (int c = 0; c < dpn; ++c)
+            for (int d = 0; d < dpn; ++d)
+              stencil[(static_cast<size_t>(j) * dpn + c) * dpn + d] +=
+                  Lay.srows_c[(static_cast<size_t>(k) * dpn + c) * KE + b * dpn + d];
+        }
+    }
+    std::vector<double> scale(in.el_scale, in.el_scale + ne);
+    if (h->prec == BFE_PREC_DOUBLE) {
+      Gr.scale = upload(scale);
+      Gr.srows = upload(Lay.srows);
+      if (Gr.pattern != 0) {
+        Gr.srows_c = upload(Lay.srows_c);
+        Gr.stencil = upload(stencil);
+      }
+    } else {
+      Gr.scale = upload(cast_vec<float>(scale));
+      Gr.srows = upload(cast_vec<float>(Lay.srows));
+      if (Gr.pattern != 0) {
+        Gr.srows_c = upload(cast_vec<float>(Lay.srows_c));
+        Gr.stencil = upload(cast_vec<float>(stencil));
+      }
+    }
+    E->groups.push_back(std::move(Gr));
+  }
+  for (int64_t r = 0; r < n; ++r)
+    if (E->row_group[r] < 0)
+      throw std::invalid_argument("element groups do not cover every DOF");
+  // interface rows of each group (group-local rows), grouped per workgroup
+  for (int32_t gi = 0; gi < n_groups; ++gi) {
+    EbeGroup& Gr = E->groups[gi];
+    std::vector<int32_t> rows, ks;
+    for (size_t k = 0; k < h->iface_host.size(); ++k) {
+      const int32_t r = h->iface_host[k];
+      if (E->row_group[r] == gi) {
+        rows.push_back(E->row_local[r]);
+        ks.push_back(static_cast<int32_t>(k));
+      }
+    }
+    Grouped g = group_by_block(ebe_blocks(Gr), static_cast<int64_t>(kBlock) * Gr.dpn, rows, &ks,
+                               nullptr);
+    Gr.iptr = upload(g.ptr);
+    Gr.irow = upload(g.row);
+    Gr.ik = upload(g.a);
+  }
+  // alpha and beta per node when one uniform group's nodes share them
+  const std::vector<double>& al = h->alpha;
+  bool per_node = n_groups == 1 && E->groups[0].uniform;
+  const int32_t dpn0 = E->groups[0].dpn;
+  const int64_t nn0 = E->groups[0].n_nodes;
+  for (int64_t v = 0; v < nn0 && per_node; ++v)
+    for (int c = 1; c < dpn0; ++c)
+      if (al[v * dpn0 + c] != al[v * dpn0] ||
+          (h->damped && h->beta_host[v * dpn0 + c] != h->beta_host[v * dpn0])) {
+        per_node = false;
+        break;
+      }
+  E->coef_per_node = per_node ? 1 : 0;
+  std::vector<double> alpha_e, beta_e;
+  if (per_node) {
+    alpha_e.resize(nn0);
+    for (int64_t v = 0; v < nn0; ++v) alpha_e[v] = al[v * dpn0];
+    if (h->damped) {
+      beta_e.resize(nn0);
+      for (int64_t v = 0; v < nn0; ++v) beta_e[v] = h->beta_host[v * dpn0];
+    }
+  } else {
+    alpha_e = al;
+    if (h->damped) beta_e = h->beta_host;
+  }
+  bool any_pattern = false;
+  for (const auto& g : E->groups) any_pattern = any_pattern || g.pattern != 0;
+  double err;
+  if (h->prec == BFE_PREC_DOUBLE) {
+    E->alpha = upload(alpha_e);
+    if (h->damped) E->beta = upload(beta_e);
+    err = compare_ebe_with_csr<double>(*h, *E, true);
+    if (any_pattern) err = std::max(err, compare_ebe_with_csr<double>(*h, *E, false));
+  } else {
+    E->alpha = upload(cast_vec<float>(alpha_e));
+    if (h->damped) E->beta = upload(cast_vec<float>(beta_e));
+    err = compare_ebe_with_csr<float>(*h, *E, true);
+    if (any_pattern) err = std::max(err, compare_ebe_with_csr<float>(*h, *E, false));
+  }
+  E->check_err = err;
+  const double tol = check_tol > 0.0 ? check_tol : (h->prec == BFE_PREC_DOUBLE ? 1e-9 : 1e-3);
+  if (!(err <= tol))
+    throw std::invalid_argument("element operator does not reproduce K (relative step "
+                                "difference " + std::to_string(err) + ")");
+  h->ebe = std::move(E);
+  h->use_ebe = 1;
+}
+
+}  // namespace
+
 extern "C" {
 
 bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
@@ -1895,141 +2191,28 @@ bfe_status bfe_run_batched(bfe_handle h, int32_t n_t, int32_t n_src, const int32
   });
 }
 
+bfe_status bfe_set_element_groups(bfe_handle h, int32_t n_groups, const bfe_element_group* groups,
+                                  double check_tol) {
+  return guarded([&] { set_element_groups(h, n_groups, groups, check_tol); });
+}
+
 bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, int64_t n_el,
                                     int32_t npe, const int32_t* el_nodes, const int32_t* el_tmpl,
                                     const double* el_scale, int32_t n_tmpl, const double* tmpl,
                                     double check_tol) {
   return guarded([&] {
-    if (h == nullptr) throw std::invalid_argument("NULL handle");
-    if (!((npe == 3 && dpn >= 1 && dpn <= 3) || (npe == 4 && dpn == 3)))
-      throw std::invalid_argument("element operator: supported (nodes, DOF per node) are "
-                                  "(3, 1), (3, 2), (3, 3) and (4, 3)");
-    if (n_nodes < 1 || n_nodes * dpn != h->n)
-      throw std::invalid_argument("element operator: n must equal n_nodes * dof_per_node "
-                                  "(node-major numbering of every node's DOFs)");
-    if (n_el < 1 || n_el > INT32_MAX) throw std::invalid_argument("element count out of range");
-    if (n_tmpl < 1) throw std::invalid_argument("no element templates");
-    if (!el_nodes || !el_tmpl || !el_scale || !tmpl)
-      throw std::invalid_argument("NULL element operator array");
-    const int KE = npe * dpn;
-    for (int64_t e = 0; e < n_el; ++e) {
-      if (el_tmpl[e] < 0 || el_tmpl[e] >= n_tmpl)
-        throw std::invalid_argument("element template index out of range");
-      if (!std::isfinite(el_scale[e])) throw std::invalid_argument("non-finite element scale");
-      for (int l = 0; l < npe; ++l) {
-        const int32_t v = el_nodes[e * npe + l];
-        if (v < 0 || v >= n_nodes) throw std::invalid_argument("element node index out of range");
-      }
-    }
-    for (int64_t i = 0; i < static_cast<int64_t>(n_tmpl) * KE * KE; ++i)
-      if (!std::isfinite(tmpl[i])) throw std::invalid_argument("non-finite template entry");
-    bfe::EbeLayout Lay = bfe::build_ebe_layout(n_nodes, dpn, n_el, npe, el_nodes, el_tmpl,
-                                                n_tmpl, tmpl, kEbeSlotPad, 64);
-    std::vector<double> scale(el_scale, el_scale + n_el);
-    BFE_HIP(hipSetDevice(h->device));
-    auto E = std::make_unique<EbeOp>();
-    E->n_nodes = n_nodes;
-    E->n_el = n_el;
-    E->dpn = dpn;
-    E->npe = npe;
-    E->n_slot = Lay.n_slot_padded;
-    E->n_tmpl = n_tmpl;
-    E->n_cls = Lay.n_cls;
-    E->pattern = match_pattern(Lay);
-    if (E->pattern != 0 && !has_pattern_kernel(*E)) E->pattern = 0;
-    E->adj = upload(Lay.adj);
-    E->nodes = upload(Lay.nodes4);
-    if (E->pattern != 0) {
-      E->nbr = upload(Lay.nbr);
-      E->cls_off = Lay.cls_off;
-      // slots whose element scale is not exactly 1 (absorbing layer)
-      std::vector<uint32_t> sm(n_nodes, 0u);
-      for (int32_t k = 0; k < Lay.n_slot; ++k)
-        for (int64_t v = 0; v < n_nodes; ++v) {
-          const int32_t e = Lay.adj[static_cast<size_t>(k) * n_nodes + v];
-          if (e >= 0 && el_scale[e] != 1.0) sm[v] |= 1u << k;
-        }
-      E->pmask = upload(Lay.present);
-      E->smask = upload(sm);
-    }
-    // summed stencil of the pattern: B_j = sum over (slot, vertex) of class j
-    std::vector<double> stencil;
-    if (E->pattern != 0) {
-      const int ke = npe * dpn;
-      stencil.assign(static_cast<size_t>(Lay.n_cls) * dpn * dpn, 0.0);
-      for (int32_t k = 0; k < Lay.n_slot; ++k)
-        for (int b = 0; b < npe; ++b) {
-          const int32_t j = Lay.cls[k * npe + b];
-          for (int c = 0; c < dpn; ++c)
-            for (int d = 0; d < dpn; ++d)
-              stencil[(static_cast<size_t>(j) * dpn + c) * dpn + d] +=
-                  Lay.srows_c[(static_cast<size_t>(k) * dpn + c) * ke + b * dpn + d];
-        }
-    }
-    // alpha and beta per node when every node's DOFs share them
-    bool per_node = true;
-    const std::vector<double>& al = h->alpha;
-    for (int64_t v = 0; v < n_nodes && per_node; ++v)
-      for (int c = 1; c < dpn; ++c)
-        if (al[v * dpn + c] != al[v * dpn]) { per_node = false; break; }
-    if (per_node && h->damped) {
-      for (int64_t v = 0; v < n_nodes && per_node; ++v)
-        for (int c = 1; c < dpn; ++c)
-          if (h->beta_host[v * dpn + c] != h->beta_host[v * dpn]) { per_node = false; break; }
-    }
-    E->coef_per_node = per_node ? 1 : 0;
-    std::vector<double> alpha_e, beta_e;
-    if (per_node) {
-      alpha_e.resize(n_nodes);
-      for (int64_t v = 0; v < n_nodes; ++v) alpha_e[v] = al[v * dpn];
-      if (h->damped) {
-        beta_e.resize(n_nodes);
-        for (int64_t v = 0; v < n_nodes; ++v) beta_e[v] = h->beta_host[v * dpn];
-      }
-    } else {
-      alpha_e = al;
-      if (h->damped) beta_e = h->beta_host;
-    }
-    const int64_t nb = (n_nodes + kBlock - 1) / kBlock;
-    std::vector<int32_t> ks(h->iface_host.size());
-    for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
-    Grouped gi = group_by_block(nb, static_cast<int64_t>(kBlock) * dpn, h->iface_host, &ks, nullptr);
-    E->iptr = upload(gi.ptr);
-    E->irow = upload(gi.row);
-    E->ik = upload(gi.a);
-    const std::vector<double>& srows = Lay.srows;
-    const std::vector<double>& srows_c = Lay.srows_c;
-    double err;
-    if (h->prec == BFE_PREC_DOUBLE) {
-      E->scale = upload(scale);
-      E->srows = upload(srows);
-      if (E->pattern != 0) {
-        E->srows_c = upload(srows_c);
-        E->stencil = upload(stencil);
-      }
-      E->alpha = upload(alpha_e);
-      if (h->damped) E->beta = upload(beta_e);
-      err = compare_ebe_with_csr<double>(*h, *E, true);
-      if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<double>(*h, *E, false));
-    } else {
-      E->scale = upload(cast_vec<float>(scale));
-      E->srows = upload(cast_vec<float>(srows));
-      if (E->pattern != 0) {
-        E->srows_c = upload(cast_vec<float>(srows_c));
-        E->stencil = upload(cast_vec<float>(stencil));
-      }
-      E->alpha = upload(cast_vec<float>(alpha_e));
-      if (h->damped) E->beta = upload(cast_vec<float>(beta_e));
-      err = compare_ebe_with_csr<float>(*h, *E, true);
-      if (E->pattern != 0) err = std::max(err, compare_ebe_with_csr<float>(*h, *E, false));
-    }
-    E->check_err = err;
-    const double tol = check_tol > 0.0 ? check_tol : (h->prec == BFE_PREC_DOUBLE ? 1e-9 : 1e-3);
-    if (!(err <= tol))
-      throw std::invalid_argument("element operator does not reproduce K (relative step "
-                                  "difference " + std::to_string(err) + ")");
-    h->ebe = std::move(E);
-    h->use_ebe = 1;
+    bfe_element_group g{};
+    g.n_nodes = n_nodes;
+    g.dpn = dpn;
+    g.npe = npe;
+    g.n_el = n_el;
+    g.el_nodes = el_nodes;
+    g.el_tmpl = el_tmpl;
+    g.el_scale = el_scale;
+    g.n_tmpl = n_tmpl;
+    g.tmpl = tmpl;
+    g.node_dof = nullptr;  // node j's DOFs are j*dpn + c
+    set_element_groups(h, 1, &g, check_tol);
   });
 }
 
@@ -2094,12 +2277,19 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
     I.device = h->device;
     I.device_bytes = h->device_bytes();
     I.operator_kind = (h->ebe && h->use_ebe) ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
-    I.ebe_slots = h->ebe ? h->ebe->n_slot : 0;
-    I.ebe_templates = h->ebe ? h->ebe->n_tmpl : 0;
-    I.ebe_elements = h->ebe ? h->ebe->n_el : 0;
-    I.ebe_check = h->ebe ? h->ebe->check_err : 0.0;
-    I.ebe_pattern = h->ebe ? h->ebe->pattern : 0;
-    I.ebe_classes = h->ebe ? h->ebe->n_cls : 0;
+    if (h->ebe) {
+      // totals over the groups; pattern and classes of the first group
+      const EbeOp& E = *h->ebe;
+      for (const auto& g : E.groups) {
+        I.ebe_slots += g.n_slot;
+        I.ebe_templates += g.n_tmpl;
+      }
+      I.ebe_elements = E.n_el();
+      I.ebe_check = E.check_err;
+      I.ebe_pattern = E.groups[0].pattern;
+      I.ebe_classes = E.groups[0].n_cls;
+      I.ebe_groups = static_cast<int32_t>(E.groups.size());
+    }
     *info = I;
   });
 }

@@ -13,9 +13,13 @@ per-element scale:
 ``element_operator`` finds the templates (shapes are compared after rounding
 the node offsets to 2^-24 of the largest element extent) and returns the
 arrays that ``Plan.set_element_operator`` hands to the library, which checks
-one step against the CSR operator before using it. Returns None when the
-model does not have that form: more than one element type, DOFs not numbered
-node-major with the same DOFs on every node, or too many templates.
+one step against the CSR operator before using it. A mesh of one element
+type with every node carrying the same DOFs gives one group over all nodes;
+a mesh of several types (solid CPE3 and fluid AC2D3 with ASI2D2 interface
+elements, which have no stiffness) gives one group per type, each over its
+own nodes with each node's first global DOF. Returns None when the model
+does not have that form (a node's DOFs of one group not consecutive, too
+many templates, unsupported element types).
 """
 from __future__ import annotations
 
@@ -33,16 +37,36 @@ _MAX_SLOTS = 64
 
 @dataclass
 class ElementOperator:
+    """One element group: elements of one type over the group's own nodes
+    0..n_nodes-1. node_dof[j] is the global DOF (0-based) of component 0 of
+    node j; None means j*dpn (a single group over every node)."""
     n_nodes: int
     dpn: int
-    el_nodes: np.ndarray   # (n_el, npe) int32, 0-based
+    el_nodes: np.ndarray   # (n_el, npe) int32, 0-based group-local nodes
     el_tmpl: np.ndarray    # (n_el,) int32
     el_scale: np.ndarray   # (n_el,) float64
     tmpl: np.ndarray       # (n_tmpl, npe*dpn, npe*dpn) float64
+    node_dof: np.ndarray | None = None
+    typ: str = ""
 
     @property
     def n_el(self) -> int:
         return int(self.el_nodes.shape[0])
+
+    @property
+    def groups(self):
+        return [self]
+
+
+@dataclass
+class ElementGroups:
+    """Element operator of a model whose nodes carry different DOF sets: one
+    ElementOperator per element type, K = sum of the groups."""
+    groups: list
+
+    @property
+    def n_el(self) -> int:
+        return sum(g.n_el for g in self.groups)
 
 
 def _unique_type(el_typ_i):
@@ -52,45 +76,11 @@ def _unique_type(el_typ_i):
     return a[0]
 
 
-def element_operator(mod, matls, fe_options, gl_lookup, n_dof):
-    """ElementOperator for the model, or None if it does not qualify."""
-    o = set_default_fields(dict(fe_options or {}), {
-        "dof_to_use": [], "damping_power_law": 3, "max_stiffness_reduction": 0.01})
-    if "el_typ_i" in mod:
-        typ = _unique_type(mod["el_typ_i"])
-    else:
-        typ = _unique_type([matls[i - 1]["el_typ"] for i in np.unique(mod["el_mat_i"])])
-    if typ not in _NPE:
-        return None
-    npe = _NPE[typ]
-    fn = element_function(typ)
-    ln, ld = fn(None, None, None, None, o["dof_to_use"])
-    ln, ld = np.asarray(ln), np.asarray(ld)
-    if ln.size == 0 or ln.size % npe:
-        return None
-    dpn = ln.size // npe
-    dofs = ld[:dpn]
-    # local DOF order must be node-major with the same DOFs on every vertex
-    if not (np.array_equal(ln, np.repeat(np.arange(1, npe + 1), dpn))
-            and np.array_equal(ld, np.tile(dofs, npe))):
-        return None
-    nds = np.asarray(mod["nds"], dtype=np.float64)
-    n_nodes = nds.shape[0]
-    gl = np.asarray(gl_lookup)
-    if n_dof != n_nodes * dpn or gl.shape[0] != n_nodes:
-        return None
-    expect = np.arange(n_nodes, dtype=np.int64)[:, None] * dpn + np.arange(dpn) + 1
-    if gl.shape[1] < int(dofs.max()) or not np.array_equal(gl[:, dofs - 1], expect):
-        return None
-    if np.count_nonzero(gl) != n_nodes * dpn:
-        return None
-    els = np.asarray(mod["els"])[:, :npe]
-    n_el = els.shape[0]
-    mat = np.asarray(mod["el_mat_i"], dtype=np.int64).ravel()
-    if mat.size != n_el or mat.min() < 1:
-        return None
-    el_nodes = (els - 1).astype(np.int32)
-    # shape keys: node offsets from vertex 1, rounded; material in the key
+def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
+    """(el_tmpl, tmpl) of a set of elements of one type, or None when there
+    are too many distinct shapes. Shapes are node offsets from vertex 1
+    rounded to 2^-24 of the largest element extent, with the material."""
+    n_el = el_nodes.shape[0]
     ext = 0.0
     for s0 in range(0, n_el, _CHUNK):
         e = el_nodes[s0:s0 + _CHUNK]
@@ -116,30 +106,100 @@ def element_operator(mod, matls, fe_options, gl_lookup, n_dof):
     for k, s0 in enumerate(range(0, n_el, _CHUNK)):  # no hash collisions
         if not np.array_equal(qs[k], rep_q[inv[s0:s0 + qs[k].shape[0]]]):
             return None
-    del qs
     from .assembly import _material
     tmpl = np.zeros((uk.size, npe * dpn, npe * dpn))
     for t, f in enumerate(first):
         D, rho = _material(matls, int(mat[f]))
-        Ke = fn(nds, el_nodes[f:f + 1].astype(np.int64), D, rho, o["dof_to_use"])[0]
-        tmpl[t] = Ke[0]
-    a = np.asarray(mod.get("el_abs_i", np.zeros(n_el)), dtype=np.float64).ravel()
+        tmpl[t] = fn(nds, el_nodes[f:f + 1].astype(np.int64), D, rho, dof_to_use)[0][0]
+    return inv.astype(np.int32), tmpl
+
+
+def element_operator(mod, matls, fe_options, gl_lookup, n_dof):
+    """ElementOperator (one group over every node, node-major numbering) or
+    ElementGroups (one group per element type with stiffness) for the model,
+    or None if it does not qualify."""
+    o = set_default_fields(dict(fe_options or {}), {
+        "dof_to_use": [], "damping_power_law": 3, "max_stiffness_reduction": 0.01})
+    els_all = np.asarray(mod["els"])
+    n_all = els_all.shape[0]
+    if "el_typ_i" in mod:
+        typ_i = np.asarray(mod["el_typ_i"], dtype=object).ravel()
+    else:
+        typ_i = np.asarray([matls[i - 1]["el_typ"] for i in np.asarray(mod["el_mat_i"])], dtype=object)
+    one = _unique_type(typ_i)
+    types = [one] if one is not None else sorted(set(typ_i.tolist()))
+    types = [t for t in types if t != "ASI2D2"]  # interface elements have no stiffness (C only)
+    if not types or any(t not in _NPE for t in types):
+        return None
+    nds = np.asarray(mod["nds"], dtype=np.float64)
+    n_nodes_all = nds.shape[0]
+    gl = np.asarray(gl_lookup)
+    mat_all = np.asarray(mod["el_mat_i"], dtype=np.int64).ravel()
+    a_all = np.asarray(mod.get("el_abs_i", np.zeros(n_all)), dtype=np.float64).ravel()
     p = float(o["damping_power_law"])
-    el_scale = np.exp(np.log(float(o["max_stiffness_reduction"])) * a ** (p + 1))
-    return ElementOperator(n_nodes=n_nodes, dpn=dpn, el_nodes=el_nodes,
-                           el_tmpl=inv.astype(np.int32), el_scale=el_scale, tmpl=tmpl)
+    lmsr = np.log(float(o["max_stiffness_reduction"]))
+    groups = []
+    for typ in types:
+        npe = _NPE[typ]
+        fn = element_function(typ)
+        ln, ld = fn(None, None, None, None, o["dof_to_use"])
+        ln, ld = np.asarray(ln), np.asarray(ld)
+        if ln.size == 0 or ln.size % npe:
+            return None
+        dpn = ln.size // npe
+        dofs = ld[:dpn]
+        # local DOF order must be node-major with the same DOFs on every vertex
+        if not (np.array_equal(ln, np.repeat(np.arange(1, npe + 1), dpn))
+                and np.array_equal(ld, np.tile(dofs, npe))):
+            return None
+        sel = np.nonzero(typ_i == typ)[0] if one is None else np.arange(n_all)
+        els = els_all[sel, :npe] - 1
+        mat = mat_all[sel]
+        if mat.size and mat.min() < 1:
+            return None
+        if gl.shape[1] < int(dofs.max()):
+            return None
+        if len(types) == 1 and n_dof == n_nodes_all * dpn and gl.shape[0] == n_nodes_all:
+            expect = np.arange(n_nodes_all, dtype=np.int64)[:, None] * dpn + np.arange(dpn) + 1
+            if (np.array_equal(gl[:, dofs - 1], expect)
+                    and np.count_nonzero(gl) == n_nodes_all * dpn):
+                nodes, el_nodes, node_dof = None, els.astype(np.int32), None
+            else:
+                nodes = np.unique(els)
+        else:
+            nodes = np.unique(els)
+        if nodes is not None:
+            # the group's nodes in global order; a node's DOFs must be consecutive
+            g0 = gl[nodes][:, dofs - 1].astype(np.int64)
+            if np.any(g0 <= 0) or np.any(g0 - g0[:, :1] != np.arange(dpn)):
+                return None
+            node_dof = (g0[:, 0] - 1).astype(np.int32)
+            loc = np.full(n_nodes_all, -1, dtype=np.int64)
+            loc[nodes] = np.arange(nodes.size)
+            el_nodes = loc[els].astype(np.int32)
+        tm = _templates(nds, els.astype(np.int32), mat, matls, fn, o["dof_to_use"], npe, dpn)
+        if tm is None:
+            return None
+        el_tmpl, tmpl = tm
+        el_scale = np.exp(lmsr * a_all[sel] ** (p + 1))
+        groups.append(ElementOperator(n_nodes=int(n_nodes_all if nodes is None else nodes.size),
+                                      dpn=dpn, el_nodes=el_nodes, el_tmpl=el_tmpl,
+                                      el_scale=el_scale, tmpl=tmpl, node_dof=node_dof, typ=typ))
+    if len(groups) == 1 and groups[0].node_dof is None:
+        return groups[0]
+    return ElementGroups(groups)
 
 
-def ebe_bytes_per_dof(op: ElementOperator, n: int, damped: bool) -> float:
+def ebe_bytes_per_dof(op, n: int, damped: bool) -> float:
     """Algorithmic HBM bytes per DOF-step of the element-by-element step:
     per element its node indices (4 npe), template index (1) and scale (8);
     per DOF u_n and u_{n-1} read, u_{n+1} written (24) and the coefficient
     streams (alpha, plus beta when damped)."""
-    npe = op.el_nodes.shape[1]
-    return op.n_el / n * (4 * npe + 1 + 8) + 24 + 8 * (2 if damped else 1)
+    el = sum(g.n_el * (4 * g.el_nodes.shape[1] + 1 + 8) for g in op.groups)
+    return el / n + 24 + 8 * (2 if damped else 1)
 
 
-def ebe_stored_bytes_per_dof(op: ElementOperator, n: int, n_slot: int, damped: bool,
+def ebe_stored_bytes_per_dof(op, n: int, n_slot: int, damped: bool,
                              n_classes: int = 0) -> float:
     """Bytes per DOF-step this implementation streams if every record is read
     once: the slot-major node-to-element table (4 n_slot per node), and either
@@ -151,4 +211,5 @@ def ebe_stored_bytes_per_dof(op: ElementOperator, n: int, n_slot: int, damped: b
         per_node, per_el = 4.0 * n_slot + 4.0 * (n_classes - 1), 8.0
     else:
         per_node, per_el = 4.0 * n_slot, 24.0
-    return (per_node * op.n_nodes + per_el * op.n_el) / n + 24 + 8 * (2 if damped else 1)
+    nodes = sum(g.n_nodes for g in op.groups)
+    return (per_node * nodes + per_el * op.n_el) / n + 24 + 8 * (2 if damped else 1)

@@ -68,13 +68,13 @@ typedef struct bfe_info {
   int32_t device;
   int64_t device_bytes;  /* operator + coefficient bytes on the device   */
   int32_t operator_kind; /* BFE_OPERATOR_CSR or BFE_OPERATOR_EBE (march) */
-  int32_t ebe_slots;     /* (template, vertex) slots of the element operator */
-  int32_t ebe_templates; /* element templates                            */
-  int32_t ebe_pattern;   /* mesh pattern of the register kernel, 0: none  */
+  int32_t ebe_slots;     /* (template, vertex) slots, summed over groups  */
+  int32_t ebe_templates; /* element templates, summed over groups         */
+  int32_t ebe_pattern;   /* mesh pattern of the first group, 0: none      */
   int64_t ebe_elements;  /* elements of the element operator             */
   double ebe_check;      /* set-up check: relative step difference to CSR */
   int32_t ebe_classes;   /* neighbour classes of the mesh pattern         */
-  int32_t reserved1;
+  int32_t ebe_groups;    /* element groups of the element operator       */
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
@@ -251,6 +251,35 @@ bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, 
                                     int32_t npe, const int32_t* el_nodes, const int32_t* el_tmpl,
                                     const double* el_scale, int32_t n_tmpl, const double* tmpl,
                                     double check_tol);
+
+/*
+ * bfe_set_element_groups — the element operator for meshes whose nodes carry
+ * different DOF sets (e.g. CPE3 solid nodes with u1, u2 and AC2D3 fluid
+ * nodes with p, fn_build_global_matrices_v5.m:83-128): one group per element
+ * type, each with its own node numbering 0..n_nodes-1 and
+ *   node_dof (n_nodes)  global DOF of component 0 of each node; the node's
+ *                       dpn components are node_dof[j] + c
+ * (NULL: node j's DOFs are j*dpn + c, a single group only). Every global DOF
+ * must belong to exactly one group (K has no entry between groups; the
+ * ASI2D2 coupling is in C). The other fields are those of
+ * bfe_set_element_operator; the same set-up check applies. At most 8 groups.
+ */
+typedef struct bfe_element_group {
+  int64_t n_nodes;
+  int32_t dpn;
+  int32_t npe;
+  int64_t n_el;
+  const int32_t* el_nodes;  /* n_el x npe, group-local node indices */
+  const int32_t* el_tmpl;   /* n_el                                 */
+  const double* el_scale;   /* n_el                                 */
+  int32_t n_tmpl;
+  int32_t reserved;
+  const double* tmpl;       /* n_tmpl x (npe*dpn)^2                 */
+  const int32_t* node_dof;  /* n_nodes, or NULL                     */
+} bfe_element_group;
+
+bfe_status bfe_set_element_groups(bfe_handle h, int32_t n_groups, const bfe_element_group* groups,
+                                  double check_tol);
 
 /*
  * bfe_set_kernel_timing — sample the duration of the main step kernel.

@@ -14,14 +14,19 @@ def _ebe():
 
 
 def _reassemble(op, n):
-    npe = op.el_nodes.shape[1]
-    dpn = op.dpn
-    ke = npe * dpn
-    dof = (op.el_nodes[:, :, None].astype(np.int64) * dpn + np.arange(dpn)).reshape(-1, ke)
-    vals = op.tmpl[op.el_tmpl] * op.el_scale[:, None, None]
-    r = np.broadcast_to(dof[:, :, None], vals.shape)
-    c = np.broadcast_to(dof[:, None, :], vals.shape)
-    return sp.csr_matrix((vals.ravel(), (r.ravel(), c.ravel())), shape=(n, n))
+    K = sp.csr_matrix((n, n))
+    for g in op.groups:
+        npe = g.el_nodes.shape[1]
+        dpn = g.dpn
+        ke = npe * dpn
+        base = (g.el_nodes.astype(np.int64) * dpn if g.node_dof is None
+                else g.node_dof[g.el_nodes].astype(np.int64))
+        dof = (base[:, :, None] + np.arange(dpn)).reshape(-1, ke)
+        vals = g.tmpl[g.el_tmpl] * g.el_scale[:, None, None]
+        r = np.broadcast_to(dof[:, :, None], vals.shape)
+        c = np.broadcast_to(dof[:, None, :], vals.shape)
+        K = K + sp.csr_matrix((vals.ravel(), (r.ravel(), c.ravel())), shape=(n, n))
+    return K
 
 
 @pytest.mark.parametrize("case", ["plate", "plate_abs", "tets", "plate_all_dofs"])
@@ -54,11 +59,25 @@ def test_templates_reassemble_K(case):
         assert op.el_scale.min() < 0.05
 
 
-def test_models_without_template_form_are_refused():
-    mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.25e-3)
-    opts = {"dof_to_use": [1, 2, 4]}
+@pytest.mark.parametrize("void", [False, True])
+def test_coupled_model_gives_one_group_per_element_type(void):
+    """Solid CPE3 (u1, u2) and fluid AC2D3 (p) groups over their own nodes;
+    ASI2D2 has no stiffness. The groups reassemble K and own every DOF once."""
+    blob = (PKG.mesh.fn_create_smooth_random_blob(0.5, 3, 48, 7) * 0.3e-3 + np.array([1.0e-3, 0.5e-3])
+            if void else None)
+    mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.125e-3, void=blob)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
     K, C, M, gl = assemble(mod, matls, opts)
-    assert _ebe().element_operator(mod, matls, opts, gl, K.shape[0]) is None
+    op = _ebe().element_operator(mod, matls, opts, gl, K.shape[0])
+    assert op is not None and len(op.groups) == 2
+    assert sorted((g.typ, g.dpn) for g in op.groups) == [("AC2D3", 1), ("CPE3", 2)]
+    owned = np.concatenate([(g.node_dof[:, None] + np.arange(g.dpn)).ravel() for g in op.groups])
+    np.testing.assert_array_equal(np.sort(owned), np.arange(K.shape[0]))
+    K2 = _reassemble(op, K.shape[0])
+    assert abs(K2 - K).max() <= 1e-12 * abs(K).max()
+
+
+def test_models_without_template_form_are_refused():
     # an irregular (perturbed) mesh has one template per element shape: too many
     mod, matls = solid_plate(1e-3, 1e-3, 0.1e-3)
     rng = np.random.default_rng(0)

@@ -4,7 +4,7 @@ on every monitored history and field frame (FP64), 1e-3 for FP32."""
 import numpy as np
 import pytest
 
-from conftest import PKG, assemble, rel_l2, solid_plate
+from conftest import PKG, assemble, coupled_block, rel_l2, solid_plate
 from oracle import v6 as ov6
 
 pytestmark = pytest.mark.gpu
@@ -20,6 +20,21 @@ def _mods():
 
 def _case(name):
     m = PKG.mesh
+    if name.startswith("coupled"):
+        # solid + water with ASI2D2 interface, absorbing layer, optional void;
+        # pressure source in the water and monitors in both media
+        void = (m.fn_create_smooth_random_blob(0.5, 3, 48, 7) * 0.3e-3 + np.array([1.5e-3, 0.7e-3])
+                if name == "coupled_void" else None)
+        mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3, void=void)
+        opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+        K, C, M, gl = assemble(mod, matls, opts)
+        dt = m.stable_time_step(K, M.diagonal())
+        t = np.arange(500) * dt
+        F = m.hann_toneburst(t, 5e6, 3)[None, :]
+        nd = m.fn_find_node_at_point(mod["nds"], [1.5e-3, 2.0e-3])
+        both = np.nonzero((gl[:, 1] > 0) & (gl[:, 3] > 0))[0]
+        fi = np.array([gl[nd - 1, 3] - 1, gl[both[len(both) // 3], 3] - 1])
+        return mod, matls, opts, K, C, M, gl, t, fi, F
     if name == "cube":
         from bristolfe_v2_amd import configs
         c = configs.cube_3d(el_size=1e-3, n_t=400)
@@ -46,10 +61,10 @@ def _case(name):
     return mod, matls, opts, K, C, M, gl, t, fi, F
 
 
-PATTERN = {"plate": 2, "plate3": 2, "iso": 3, "cube": 1}
+PATTERN = {"plate": 2, "plate3": 2, "iso": 3, "cube": 1, "coupled": 2, "coupled_void": 2}
 
 
-@pytest.mark.parametrize("name", ["plate", "plate3", "iso", "cube"])
+@pytest.mark.parametrize("name", ["plate", "plate3", "iso", "cube", "coupled", "coupled_void"])
 @pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("kernel", ["pattern", "generic"])
 def test_ebe_parity(name, mode, kernel):
@@ -64,6 +79,9 @@ def test_ebe_parity(name, mode, kernel):
         assert info["operator_kind"] == _lib.OPERATOR_EBE
         assert info["ebe_pattern"] == PATTERN[name]
         assert info["ebe_check"] < 1e-11
+        assert info["ebe_groups"] == (2 if name.startswith("coupled") else 1)
+        if name.startswith("coupled"):
+            assert info["n_interface"] > 0
         plan.set_tuning(_lib.TUNE_EBE_KERNEL, 1 if kernel == "pattern" else 0)
         h, fld, ti = plan.run(t.size, fi, F, hi, 50, 1)
     r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, F, hi + 1, 50, "KE", mode)
