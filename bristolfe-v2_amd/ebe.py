@@ -27,7 +27,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .elements import element_function
+from .elements import _tri_geometry, element_function
 from .options import set_default_fields
 
 _NPE = {"CPE3": 3, "AC2D3": 3, "C3D4": 4}
@@ -111,7 +111,20 @@ def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
     for t, f in enumerate(first):
         D, rho = _material(matls, int(mat[f]))
         tmpl[t] = fn(nds, el_nodes[f:f + 1].astype(np.int64), D, rho, dof_to_use)[0][0]
-    return inv.astype(np.int32), tmpl
+    # Triangle matrices are K_e = G(b, c) / J_e with J_e evaluated from absolute
+    # coordinates (x1*y2 - y1*x2 - ..., fn_el_CPE3.m:56): far from the origin it
+    # carries a cancellation error of about eps*|x|^2/h^2 that the edge vectors
+    # b, c do not. Each element's J_rep / J_e goes into its scale, so the
+    # operator keeps every element's own J and matches the assembled K to the
+    # rounding of b and c.
+    factor = np.ones(n_el)
+    if npe == 3:
+        Jrep = _tri_geometry(nds, el_nodes[first].astype(np.int64))[0]
+        for s0 in range(0, n_el, _CHUNK):
+            e = el_nodes[s0:s0 + _CHUNK].astype(np.int64)
+            Je = _tri_geometry(nds, e)[0]
+            factor[s0:s0 + e.shape[0]] = Jrep[inv[s0:s0 + e.shape[0]]] / Je
+    return inv.astype(np.int32), tmpl, factor
 
 
 def element_operator(mod, matls, fe_options, gl_lookup, n_dof):
@@ -180,8 +193,8 @@ def element_operator(mod, matls, fe_options, gl_lookup, n_dof):
         tm = _templates(nds, els.astype(np.int32), mat, matls, fn, o["dof_to_use"], npe, dpn)
         if tm is None:
             return None
-        el_tmpl, tmpl = tm
-        el_scale = np.exp(lmsr * a_all[sel] ** (p + 1))
+        el_tmpl, tmpl, factor = tm
+        el_scale = np.exp(lmsr * a_all[sel] ** (p + 1)) * factor
         groups.append(ElementOperator(n_nodes=int(n_nodes_all if nodes is None else nodes.size),
                                       dpn=dpn, el_nodes=el_nodes, el_tmpl=el_tmpl,
                                       el_scale=el_scale, tmpl=tmpl, node_dof=node_dof, typ=typ))

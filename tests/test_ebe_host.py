@@ -52,11 +52,38 @@ def test_templates_reassemble_K(case):
     assert op.tmpl.shape[0] == (6 if case == "tets" else 2)
     K2 = _reassemble(op, K.shape[0])
     assert abs(K2 - K).max() <= 1e-12 * abs(K).max()
-    # the absorbing scale is exactly 1 outside the layer
+    # outside the layer the scale is 1 (tets) or the element's J_rep / J_e
+    # (triangles, within the cancellation error of J)
     a = np.asarray(mod["el_abs_i"])
-    assert np.all(op.el_scale[a == 0] == 1.0)
+    if case == "tets":
+        assert np.all(op.el_scale[a == 0] == 1.0)
+    else:
+        assert np.abs(op.el_scale[a == 0] - 1.0).max() < 1e-8
     if case == "plate_abs":
         assert op.el_scale.min() < 0.05
+
+
+@pytest.mark.parametrize("typ", ["CPE3", "AC2D3"])
+def test_jacobian_cancellation_carried_by_the_scale(typ):
+    """Far from the origin the triangle Jacobian x1*y2 - y1*x2 - ... (as the
+    reference evaluates it) has a cancellation error of about eps*|x|^2/h^2;
+    the operator keeps each element's own J, so it matches the assembled K
+    row by row to the rounding of the edge vectors."""
+    from bristolfe_v2_amd import configs
+    m = PKG.mesh
+    mat = configs.steel() if typ == "CPE3" else configs.water()
+    mod = m.fn_rectangular_structured_mesh(np.array([[50e-3, 40e-3], [51e-3, 40e-3], [51e-3, 40.5e-3],
+                                                     [50e-3, 40.5e-3]]), 0.025e-3)
+    mod["el_typ_i"] = np.array([typ] * len(mod["el_mat_i"]), dtype=object)
+    opts = {"dof_to_use": [1, 2] if typ == "CPE3" else [4]}
+    K, C, M, gl = assemble(mod, [mat], opts)
+    op = _ebe().element_operator(mod, [mat], opts, gl, K.shape[0])
+    K2 = _reassemble(op, K.shape[0])
+    rowmax = abs(K).max(axis=1).toarray().ravel()
+    rel = abs(K2 - K).max(axis=1).toarray().ravel() / rowmax
+    assert rel.max() < 1e-12
+    # the J_e spread itself is far larger than that
+    assert np.abs(op.el_scale - 1).max() > 1e-11
 
 
 @pytest.mark.parametrize("void", [False, True])
