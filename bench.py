@@ -401,6 +401,19 @@ def main():
                      "kernel_avg_us": round(kern_avg_s * 1e6, 2), "kernel_samples": int(samples)},
         "loop_gdof_steps_per_s_wall": round(n * steps_done * args.steps / wall / 1e9, 4),
     }
+    # the same march with its time loop captured in one hipGraph (loop time
+    # from HIP events around the graph launch; wall includes capture and
+    # instantiation)
+    plan.set_tuning(7, 1)
+    w0 = time.perf_counter()
+    hg, _, _ = plan.run(n_t, fi, F, hi)
+    gw = time.perf_counter() - w0
+    gst = plan.stats()
+    plan.set_tuning(7, 0)
+    out["graph"] = {"gdof_steps_per_s": round(n * gst["steps"] / gst["loop_ms"] / 1e6, 4),
+                    "loop_ms": round(gst["loop_ms"], 3), "wall_s": round(gw, 3),
+                    "eager_loop_ms": round(float(np.mean(loops)), 3),
+                    "identical_to_eager": bool(np.array_equal(hg, hist))}
     if use_ebe:
         out["config"]["operator"] = (f"element-by-element: {op.n_el} elements in "
                                      f"{info['ebe_groups']} group(s), {info['ebe_templates']} "

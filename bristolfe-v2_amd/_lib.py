@@ -25,6 +25,7 @@ OPERATOR_EBE = 1
 TUNE_EBE_GROUP = 4
 TUNE_EBE_KERNEL = 5
 TUNE_MULTI_ROWS_PER_WAVE = 6
+TUNE_GRAPH = 7
 
 # Every symbol the header declares.
 EXPORTS = (

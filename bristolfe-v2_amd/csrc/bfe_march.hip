@@ -913,6 +913,7 @@ struct bfe_plan_s {
   int32_t use_ebe = 1;               // single-source march uses ebe when set
   int32_t ebe_group = 8;             // slots per load group of k_step_ebe (2, 4 or 8)
   int32_t ebe_kernel = 1;            // 1: pattern kernel when the mesh matches one
+  int32_t use_graph = 0;             // single-source march captured in a hipGraph
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
@@ -1437,6 +1438,31 @@ double compare_ebe_with_csr(const bfe_plan_s& H, const EbeOp& E, bool generic) {
   return amax > 0.0 ? dmax / amax : dmax;
 }
 
+// Owns a stream capture and its graph: a capture still open when an error
+// unwinds is ended, so the stream is usable again.
+struct GraphCapture {
+  hipStream_t stream = nullptr;  // set while capturing
+  hipStream_t launched = nullptr;  // stream the instantiated graph was launched on
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  GraphCapture() = default;
+  GraphCapture(const GraphCapture&) = delete;
+  GraphCapture& operator=(const GraphCapture&) = delete;
+  ~GraphCapture() {
+    if (stream) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+    }
+    if (exec) {
+      if (launched) (void)hipStreamSynchronize(launched);
+      (void)hipGraphExecDestroy(exec);
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 template <typename T>
 void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                 const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
@@ -1505,9 +1531,17 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     rec->cur = 0;
   }
   int64_t launches = 0;
-  KernelSampler sampler(H, t0, n_t - t0);
-
-  BFE_HIP(hipEventRecord(H.ev0, st));
+  // graph mode: the whole time loop is captured into one hipGraph and then
+  // launched (plain marches only; no per-kernel sampling)
+  const bool graph = H.use_graph != 0 && rec == nullptr;
+  KernelSampler sampler(H, t0, graph ? 0 : n_t - t0);
+  GraphCapture cap;
+  if (graph) {
+    BFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    cap.stream = st;
+  } else {
+    BFE_HIP(hipEventRecord(H.ev0, st));
+  }
   for (int64_t t = t0; t < n_t; ++t) {
     const int64_t prev = t - 1;
     T* hp = (q > 0 && prev >= t0) ? hist + prev * q : nullptr;
@@ -1526,14 +1560,14 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
       const int64_t last = sg.count() - 1;
       if (t >= sg.start[last]) store = rec->useg[0].as<T>() + (t - sg.start[last]) * n;
     }
-    sampler.before(t);
+    if (!graph) sampler.before(t);
     if (E)
       launch_step_ebe<T>(H, *E, &RL, nullptr, u1, u2, u, F, frc_rows, static_cast<int32_t>(t), q,
                          hp, fp, field_is_vel, dtT, IR);
     else
       launch_step<T>(H, 1, nblk, u1, u2, u, L, falpha, F, frc_rows, static_cast<int32_t>(t), q,
                      hp, fp, field_is_vel, dtT, IR, store, nullptr);
-    sampler.after(t);
+    if (!graph) sampler.after(t);
     ++launches;
     if (H.nI > 0 && H.has_xinv) {
       launch_iface<T>(H, 1, wI, u);
@@ -1552,6 +1586,15 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                          q, R.hidx.as<int32_t>(), hp, fp, field_is_vel, dtT);
       ++launches;
     }
+  }
+  if (graph) {
+    BFE_HIP(hipGetLastError());
+    cap.stream = nullptr;
+    BFE_HIP(hipStreamEndCapture(st, &cap.graph));
+    BFE_HIP(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0));
+    BFE_HIP(hipEventRecord(H.ev0, st));
+    cap.launched = st;
+    BFE_HIP(hipGraphLaunch(cap.exec, st));
   }
   finish_timing(H, sampler, n_t - t0, launches);
   if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(q) * n_t);
@@ -2237,6 +2280,9 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         if (value != 2 && value != 4 && value != 8)
           throw std::invalid_argument("element-operator load group must be 2, 4 or 8");
         h->ebe_group = value;
+        break;
+      case BFE_TUNE_GRAPH:
+        h->use_graph = value ? 1 : 0;
         break;
       case BFE_TUNE_EBE_KERNEL:
         h->ebe_kernel = value ? 1 : 0;

@@ -212,6 +212,12 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
 /* rows each wavefront of the multi-source march handles together (1, 2, 4
  * or 8; default 1): consecutive rows of one 64-row slice, interleaved */
 #define BFE_TUNE_MULTI_ROWS_PER_WAVE 6
+/* 1: bfe_run captures its whole time loop (step kernels, interface solve,
+ * last snapshot) into one hipGraph and launches that, instead of launching
+ * step by step; for small meshes whose steps are launch-latency bound.
+ * Plain marches only (not bfe_run_checkpointed); no kernel sampling.
+ * Default 0. */
+#define BFE_TUNE_GRAPH 7
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3

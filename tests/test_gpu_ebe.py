@@ -97,7 +97,7 @@ def test_ebe_and_csr_runs_agree_and_switch():
     mod, matls, opts, K, C, M, gl, t, fi, F = _case("cube")
     op = ebe.element_operator(mod, matls, opts, gl, K.shape[0])
     hi = np.arange(0, K.shape[0], 53)
-    with solver.make_plan(K, C, M, t[1] - t[0], element_op=op) as plan:
+    with solver.make_plan(K, C, M, t[1] - t[0], element_op=op, operator="ebe") as plan:
         h_e, _, _ = plan.run(t.size, fi, F, hi)
         plan.set_tuning(_lib.TUNE_OPERATOR, _lib.OPERATOR_CSR)
         assert plan.info()["operator_kind"] == _lib.OPERATOR_CSR
@@ -117,8 +117,15 @@ def test_wrong_templates_are_rejected():
     with pytest.raises(_lib.BfeError) as ei:
         solver.make_plan(K, C, M, t[1] - t[0], element_op=op, operator="ebe")
     assert ei.value.status == _lib.BFE_ERR_ARG
-    with solver.make_plan(K, C, M, t[1] - t[0], element_op=op, operator="auto") as plan:
-        assert plan.info()["operator_kind"] == _lib.OPERATOR_CSR
+    # "auto" falls back to CSR when the library rejects the element operator
+    # (the model is below the automatic size threshold, so lower it here)
+    threshold = solver.AUTO_EBE_MIN_DOF
+    solver.AUTO_EBE_MIN_DOF = 0
+    try:
+        with solver.make_plan(K, C, M, t[1] - t[0], element_op=op, operator="auto") as plan:
+            assert plan.info()["operator_kind"] == _lib.OPERATOR_CSR
+    finally:
+        solver.AUTO_EBE_MIN_DOF = threshold
 
 
 def test_ebe_single_precision_error_reported():

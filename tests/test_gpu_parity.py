@@ -228,3 +228,32 @@ def test_cube_3d_tets_parity(mode):
                                                 solver_mode=mode)
     assert np.linalg.norm(r[:16]) > 0
     assert rel_l2(h, r) < TOL
+
+
+@pytest.mark.parametrize("operator", ["csr", "ebe"])
+def test_graph_captured_march_matches_eager(operator):
+    """The time loop captured in one hipGraph gives the eager march's
+    histories and field frames bit for bit; repeated graph runs, with eager
+    runs and synchronisations between them, stay identical (coupled model:
+    interface solve in the loop; "ebe": element-by-element groups)."""
+    print("case: graph capture", operator, flush=True)
+    from bristolfe_v2_amd import _lib, ebe
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+    K, C, M, gl, t, F, fi, hi = _setup(mod, matls, opts, 400, 5e6, src=[1.5e-3, 2.0e-3], dof=4)
+    op = ebe.element_operator(mod, matls, opts, gl, K.shape[0]) if operator == "ebe" else None
+    plan = PKG.make_plan(K, C, M, t[1] - t[0], element_op=op, operator=operator)
+    try:
+        assert plan.info()["operator_kind"] == (1 if operator == "ebe" else 0)
+        ref, rfld, ti = plan.run(t.size, fi - 1, F, hi - 1, 50, 1)
+        for _ in range(3):
+            plan.set_tuning(_lib.TUNE_GRAPH, 1)
+            h, fld, ti2 = plan.run(t.size, fi - 1, F, hi - 1, 50, 1)
+            assert ti2 == ti
+            np.testing.assert_array_equal(h, ref)
+            np.testing.assert_array_equal(fld, rfld)
+            plan.set_tuning(_lib.TUNE_GRAPH, 0)
+            h2, _, _ = plan.run(t.size, fi - 1, F, hi - 1, 50, 1)
+            np.testing.assert_array_equal(h2, ref)
+    finally:
+        plan.close()
