@@ -372,18 +372,20 @@ __global__ void k_snap(int64_t n, int32_t S, const T* __restrict__ u_last,
 }
 
 // ---------------------------------------------------------------------------
-// Element-by-element step (single source). The stiffness is held as
+// Element-by-element step (single source). Within one element group the
+// stiffness is held as
 //   K = sum_e s_e P_e' T[tau_e] P_e
 // with a few element templates T (one per element shape and material, local
 // DOF order node-major) and a per-element scale s_e (the absorbing-layer
-// stiffness reduction), and the global DOF of (node v, component c) is
-// v*DPN + c. Thread v owns node v's DPN rows and gathers its row of K u from
-// the elements around it: slot k of every node holds the element in which the
-// node is local vertex l_k of template tau_k (or -1), so the DPN x (NPE*DPN)
-// template rows a slot needs are the same for every lane and are read through
-// the scalar cache. The slot-major adjacency is read coalesced; element nodes
-// and u1 are gathered. The update, forcing, interface, monitor and field
-// handling are those of k_step on the node's rows.
+// stiffness reduction times J_rep / J_e), and the global DOFs of the group's
+// node v are r0 + c, r0 = dof0[v] (v*DPN when there is no DOF table). Thread v
+// owns node v's DPN rows and gathers its row of K u from the elements around
+// it: slot k of every node holds the element in which the node is local
+// vertex l_k of template tau_k (or -1), so the DPN x (NPE*DPN) template rows a
+// slot needs are the same for every lane and are read through the scalar
+// cache. The slot-major adjacency is read coalesced; element records (each
+// vertex's first DOF) and u1 are gathered. The update, forcing, interface,
+// monitor and field handling are those of k_step on the node's rows.
 constexpr int kEbeSlotPad = 8;  // slot count padding: any load group G of 1, 2, 4, 8 divides it
 
 // acc[c] += sum over the G slots k0..k0+G-1 of s_e T_k(c, :) u1(e), given
