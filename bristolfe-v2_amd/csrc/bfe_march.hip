@@ -598,13 +598,16 @@ struct ClassOffsets {
 // (irregular nodes, flagged in bit 31 of pmask, read it from the class-major
 // neighbour table nbr). pmask bit k marks an occupied slot, smask bit k a slot
 // whose element scale differs from 1: only those read the slot table and the
-// scale. Template rows (canonical vertex order) come through the scalar cache.
-template <typename T, bool DAMPED, typename PT, int DPN>
+// scale. DIRECT groups (most slots scaled) read instead a slot-major table of
+// per-slot scales, 0 for an empty slot, with no table -> element indirection.
+// Template rows (canonical vertex order) come through the scalar cache.
+template <typename T, bool DAMPED, typename PT, int DPN, bool DIRECT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
     int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
     ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
     const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
+    const T* __restrict__ slot_scale,
     const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
     const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
     T* __restrict__ u, BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F,
@@ -618,7 +621,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   const int64_t ib = pm >> 31;  // 1: irregular node
   const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
   constexpr uint32_t kAll = (1u << NS) - 1u;
-  if (__all((pm & kAll) == kAll && sm == 0u)) {
+  if (!DIRECT && __all((pm & kAll) == kAll && sm == 0u)) {
     // every lane's elements are all present with scale 1 (the interior): K u
     // is the fixed stencil sum_j B_j u_j, B_j the sum of the template blocks
     // of neighbour class j (NC DPN x DPN blocks instead of NS template rows)
@@ -669,9 +672,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // predicates as integer lane values (0 or 1) and bit masks: kept per
     // lane in VGPRs, where 24 slots' worth of boolean lane masks would
     // exhaust the scalar registers
-    const uint32_t sb = (sm >> k) & 1u;
-    const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * sb];
-    const T s = pick_bits(el_scale[static_cast<int64_t>(e) * sb], T(1), sb);
+    T s;
+    if constexpr (DIRECT) {
+      s = slot_scale[static_cast<int64_t>(k) * n_nodes + v];  // 0 for an empty slot
+    } else {
+      const uint32_t sb = (sm >> k) & 1u;
+      const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * sb];
+      s = pick_bits(el_scale[static_cast<int64_t>(e) * sb], T(1), sb);
+    }
     const T* R = srows + k * (DPN * KE);
     T y[DPN];
 #pragma unroll
@@ -682,9 +690,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       for (int d = 0; d < DPN; ++d)
 #pragma unroll
         for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
-    const uint32_t ob = (pm >> k) & 1u;
+    if constexpr (DIRECT) {
 #pragma unroll
-    for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
+      for (int c = 0; c < DPN; ++c) acc[c] += s * y[c];
+    } else {
+      const uint32_t ob = (pm >> k) & 1u;
+#pragma unroll
+      for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
+    }
   }
   ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
                              q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
@@ -847,13 +860,14 @@ struct EbeGroup {
   int32_t dpn = 0, npe = 0, n_slot = 0, n_tmpl = 0, n_cls = 0;
   int32_t pattern = 0;       // mesh pattern of k_step_ebe_pat, 0: none
   bool uniform = false;      // node j's first DOF is j*dpn (no DOF table read)
+  bool direct = false;       // per-slot scale table instead of table -> element -> scale
   std::vector<int64_t> doff; // first-DOF offset of each neighbour class on regular nodes
-  DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil;
+  DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil, slot_scale;
   DevBuf iptr, irow, ik;     // interface rows (group-local rows) per workgroup
   int64_t bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&dof0, &adj, &nodes, &scale, &srows, &nbr, &srows_c, &pmask, &smask,
-                            &stencil, &iptr, &irow, &ik})
+                            &stencil, &slot_scale, &iptr, &irow, &ik})
       s += static_cast<int64_t>(b->bytes);
     return s;
   }
@@ -1263,12 +1277,22 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int
                     int32_t fld_is_vel, T dt, const IfaceRows<T>& IR) {
   ClassOffsets<PT::NC> co;
   for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(Gr.doff[j]);
-  hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN>), dim3(nblk), dim3(kBlock), 0, H.stream,
-                     Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), co,
-                     Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
-                     Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.srows_c.as<T>(),
-                     Gr.stencil.as<T>(), E.alpha.as<T>(), D ? E.beta.as<T>() : nullptr, u1, u2,
-                     u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
+  if (Gr.direct)
+    hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, true>), dim3(nblk), dim3(kBlock), 0,
+                       H.stream, Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(),
+                       co, Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
+                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.slot_scale.as<T>(),
+                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha.as<T>(),
+                       D ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                       fp, fld_is_vel, dt, IR, E.coef_per_node);
+  else
+    hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, false>), dim3(nblk), dim3(kBlock), 0,
+                       H.stream, Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(),
+                       co, Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
+                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.slot_scale.as<T>(),
+                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha.as<T>(),
+                       D ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                       fp, fld_is_vel, dt, IR, E.coef_per_node);
 }
 
 // Whether a kernel exists for the group's shape (generic) or pattern.
@@ -1975,6 +1999,24 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
       Gr.nbr = upload(nbrdof);
       Gr.pmask = upload(pm);
       Gr.smask = upload(sm);
+      // most occupied slots scaled (e.g. every triangle, whose scale carries
+      // J_rep / J_e): a slot-major scale table beats the table -> element chain
+      int64_t occupied = 0, scaled = 0;
+      for (int64_t v = 0; v < nn; ++v) {
+        occupied += __builtin_popcount(pm[v] & 0x7fffffffu);
+        scaled += __builtin_popcount(sm[v]);
+      }
+      Gr.direct = 2 * scaled > occupied;
+      if (Gr.direct) {
+        std::vector<double> ss(static_cast<size_t>(Lay.n_slot) * nn, 0.0);
+        for (int32_t k = 0; k < Lay.n_slot; ++k)
+          for (int64_t v = 0; v < nn; ++v) {
+            const int32_t e = Lay.adj[static_cast<size_t>(k) * nn + v];
+            if (e >= 0) ss[static_cast<size_t>(k) * nn + v] = in.el_scale[e];
+          }
+        if (h->prec == BFE_PREC_DOUBLE) Gr.slot_scale = upload(ss);
+        else Gr.slot_scale = upload(cast_vec<float>(ss));
+      }
       // summed stencil: B_j = sum over (slot, vertex) of class j
       stencil.assign(static_cast<size_t>(nc) * dpn * dpn, 0.0);
       for (int32_t k = 0; k < Lay.n_slot; ++k)
