@@ -333,8 +333,21 @@ def main():
     hi = np.asarray(c["hist_gi"]) - 1
     F = c["frcs"]
 
+    # field frames of the workload (the plate: every 100 steps), reduced to the
+    # per-element 'KE' field on the device next to the march
+    fe_every = c["fe_options"].get("field_output_every_n_frames", float("inf"))
+    fe_every = int(fe_every) if np.isfinite(fe_every) else 0
+    if fe_every:
+        plan.set_field_elements(c["mod"]["els"], c["gl_lookup"])
+
+    def march():
+        if fe_every:
+            h, _, ti = plan.run_field_ke(n_t, fi, F, hi, fe_every)
+            return h, None, ti
+        return plan.run(n_t, fi, F, hi)
+
     for w in range(args.warmup):
-        plan.run(n_t, fi, F, hi)
+        march()
         log(f"warmup {w + 1}/{args.warmup}: loop {plan.stats()['loop_ms']:.1f} ms")
     plan.set_kernel_timing(args.timing_every)
     # bfe_run returns after its stream has drained (hipEventSynchronize), so
@@ -343,7 +356,7 @@ def main():
     loops, kern, samples, steps_done = [], 0.0, 0, 0
     hist = None
     for s in range(args.steps):
-        hist, _, ti = plan.run(n_t, fi, F, hi)
+        hist, _, ti = march()
         st = plan.stats()
         loops.append(st["loop_ms"])
         kern += st["step_kernel_ms"]
@@ -391,7 +404,10 @@ def main():
         "config": {"workload": c["name"], "n_dof": int(n), "nnz_K": int(info["nnz_k"]),
                    "time_steps": int(n_t), "steps_marched": int(steps_done),
                    "interface_rows": int(info["n_interface"]), "monitors": int(len(hi)),
-                   "step": "one full march of all time steps (bfe_run)"},
+                   "field_frames_every": fe_every or None,
+                   "step": ("one full march of all time steps (bfe_run)" if not fe_every else
+                            "one full march of all time steps with the per-element KE field "
+                            "every %d steps (bfe_run_field_ke)" % fe_every)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,

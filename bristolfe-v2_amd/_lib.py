@@ -31,7 +31,7 @@ TUNE_GRAPH = 7
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
     "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
-    "bfe_set_element_groups",
+    "bfe_set_element_groups", "bfe_set_field_elements", "bfe_run_field_ke",
     "bfe_set_kernel_timing",
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
@@ -119,6 +119,12 @@ def load() -> C.CDLL:
     lib.bfe_set_element_operator.restype = C.c_int
     lib.bfe_set_element_groups.argtypes = [C.c_void_p, C.c_int32, _p(BfeElementGroup), C.c_double]
     lib.bfe_set_element_groups.restype = C.c_int
+    lib.bfe_set_field_elements.argtypes = [C.c_void_p, C.c_int64, C.c_int32, _i32p, C.c_int64,
+                                           C.c_int32, _i32p]
+    lib.bfe_set_field_elements.restype = C.c_int
+    lib.bfe_run_field_ke.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32,
+                                     _i32p, C.c_int32, _f64p, C.c_int32, _f64p, _i32p]
+    lib.bfe_run_field_ke.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -261,6 +267,37 @@ class Plan:
             self._h, int(op.n_nodes), int(op.dpn), n_el, npe, ptr(el_nodes, C.c_int32),
             ptr(el_tmpl, C.c_int32), ptr(el_scale, C.c_double), int(tmpl.shape[0]),
             ptr(tmpl, C.c_double), float(check_tol)))
+
+    def set_field_elements(self, els, gl_lookup):
+        """Mesh tables of the device 'KE' field: els (n_el, npe) and gl_lookup
+        (n_nodes, max_df) 1-based with 0 = absent, as the model holds them."""
+        els0 = np.ascontiguousarray(np.asarray(els, np.int64) - 1, np.int32)
+        nd0 = np.ascontiguousarray(np.asarray(gl_lookup, np.int64) - 1, np.int32)
+        self.n_field_el = els0.shape[0]
+        check(self._lib.bfe_set_field_elements(self._h, nd0.shape[0], nd0.shape[1],
+                                               ptr(nd0, C.c_int32), els0.shape[0], els0.shape[1],
+                                               ptr(els0, C.c_int32)))
+
+    def run_field_ke(self, n_t, frc_idx, frcs, hist_idx, field_every):
+        """bfe_run with the 'KE' field reduced on the device: returns (hist,
+        el_ke (n_el x n_frames), ti_start)."""
+        frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
+        hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
+        p, q = frc_idx.size, hist_idx.size
+        frcs = np.asarray(frcs, np.float64)
+        if frcs.ndim == 1:
+            frcs = frcs[None, :]
+        rows = frcs.shape[0] if p > 0 else 0
+        Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
+        hist = np.empty((q, n_t), order="F")  # the library writes every entry
+        n_frames = (n_t - 1) // int(field_every) + 1
+        el = np.empty((self.n_field_el, n_frames), order="F")
+        ti = C.c_int32()
+        check(self._lib.bfe_run_field_ke(
+            self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
+            ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(field_every),
+            el.ctypes.data_as(_f64p), C.byref(ti)))
+        return hist, el, int(ti.value)
 
     def set_element_groups(self, groups, check_tol=0.0):
         keep = []

@@ -722,6 +722,36 @@ int match_pattern(const bfe::EbeLayout& L) {
 }
 
 // ---------------------------------------------------------------------------
+// Per-element kinetic-energy field of code/internal/fn_get_field_output.m:64-79
+// from velocity frames (n x F): KE_node = sum over the node's DOFs of
+// |v^2 m| / 2, then the mean over the element's nodes (absent: -1), then the
+// square root. One thread per (element, frame).
+template <typename T>
+__global__ void k_field_ke(int64_t n_el, int32_t npe, const int32_t* __restrict__ els,
+                           int32_t max_df, const int32_t* __restrict__ node_dof,
+                           const T* __restrict__ m, const T* __restrict__ fld, int64_t n,
+                           int32_t F, T* __restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_el * F) return;
+  const int64_t e = i % n_el, f = i / n_el;
+  const T* v = fld + f * n;
+  T sum = T(0);
+  int cnt = 0;
+  for (int a = 0; a < npe; ++a) {
+    const int32_t node = els[e * npe + a];
+    if (node < 0) continue;
+    ++cnt;
+    T ke = T(0);
+    for (int d = 0; d < max_df; ++d) {
+      const int32_t g = node_dof[static_cast<int64_t>(node) * max_df + d];
+      if (g >= 0) ke += fabs(v[g] * v[g] * m[g]) / T(2);
+    }
+    sum += ke;
+  }
+  out[i] = cnt > 0 ? sqrt(sum / T(cnt)) : T(0);
+}
+
+// ---------------------------------------------------------------------------
 // Per-element gradients over one checkpoint segment of Ls time steps.
 // Thread e keeps K_e (KMAX x KMAX), the lumped mass per unit density and the
 // last two forward states of its DOFs in registers and walks the segment:
@@ -925,6 +955,8 @@ struct bfe_plan_s {
   std::shared_ptr<void> grad_elems;  // GradElements
   std::vector<int32_t> iface_host;   // interface rows (global), for regrouping
   std::vector<double> beta_host;     // beta per row (damped plans), for the element operator
+  std::vector<double> m_host;        // lumped mass per DOF (field output)
+  std::shared_ptr<void> field_elems; // FieldElements
   std::unique_ptr<EbeOp> ebe;        // optional element-by-element operator
   int32_t use_ebe = 1;               // single-source march uses ebe when set
   int32_t ebe_group = 8;             // slots per load group of k_step_ebe (2, 4 or 8)
@@ -937,6 +969,7 @@ struct bfe_plan_s {
     }
     adj_records.clear();
     grad_elems.reset();
+    field_elems.reset();
     ebe.reset();
     for (DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr, &xcol,
                       &xval, &is_ptr, &is_row, &is_k})
@@ -1489,15 +1522,26 @@ struct GraphCapture {
   }
 };
 
+// Element and node tables of the per-element field output.
+struct FieldElements {
+  int64_t n_el = 0, n_nodes = 0;
+  int32_t npe = 0, max_df = 0;
+  DevBuf els, node_dof, m;
+};
+
 template <typename T>
 void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                 const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
                 double* hist_out, int32_t field_every, int32_t field_is_vel,
-                double* field_out, int32_t* ti_start_out, AdjointRecord* rec, int32_t ckpt_every) {
+                double* field_out, int32_t* ti_start_out, AdjointRecord* rec, int32_t ckpt_every,
+                double* el_ke_out = nullptr) {
   const int64_t n = H.n;
   validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   const int32_t n_frames = field_every > 0 ? (n_t - 1) / field_every + 1 : 0;
-  if (n_frames > 0 && field_out == nullptr) throw std::invalid_argument("field output is NULL");
+  auto FE = std::static_pointer_cast<FieldElements>(H.field_elems);
+  if (el_ke_out != nullptr && !FE) throw std::invalid_argument("field elements not set");
+  if (n_frames > 0 && field_out == nullptr && el_ke_out == nullptr)
+    throw std::invalid_argument("field output is NULL");
 
   H.stats = bfe_run_stats{};
   H.stats.columns = 1;
@@ -1517,8 +1561,10 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   }
   if (t0 < 0) {  // empty loop (v6:163-180): outputs are zero
     if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * n_t);
-    if (n_frames > 0)
+    if (n_frames > 0 && field_out != nullptr)
       std::memset(field_out, 0, sizeof(double) * static_cast<size_t>(n) * n_frames);
+    if (n_frames > 0 && el_ke_out != nullptr)
+      std::memset(el_ke_out, 0, sizeof(double) * static_cast<size_t>(FE->n_el) * n_frames);
     return;
   }
 
@@ -1624,7 +1670,19 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   }
   finish_timing(H, sampler, n_t - t0, launches);
   if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(q) * n_t);
-  if (n_frames > 0) copy_out<T>(dfld, field_out, static_cast<size_t>(n) * n_frames);
+  if (n_frames > 0 && field_out != nullptr)
+    copy_out<T>(dfld, field_out, static_cast<size_t>(n) * n_frames);
+  if (n_frames > 0 && el_ke_out != nullptr) {
+    // per-element field on the device: only n_el x F values come back
+    const int64_t work = FE->n_el * n_frames;
+    DevBuf dke(sizeof(T) * static_cast<size_t>(work));
+    hipLaunchKernelGGL((k_field_ke<T>), dim3((work + 255) / 256), dim3(256), 0, st, FE->n_el,
+                       FE->npe, FE->els.as<int32_t>(), FE->max_df, FE->node_dof.as<int32_t>(),
+                       FE->m.as<T>(), fld, n, n_frames, dke.as<T>());
+    BFE_HIP(hipGetLastError());
+    BFE_HIP(hipStreamSynchronize(st));
+    copy_out<T>(dke, el_ke_out, static_cast<size_t>(work));
+  }
 }
 
 // Element data for the parameter gradients.
@@ -2161,6 +2219,7 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->sr = P.sr;
     H->alpha = P.alpha;
     H->iface_host = P.iface;
+    H->m_host.assign(m, m + n);
     if (P.damped) H->beta_host = P.beta;
     if (precision == BFE_PREC_DOUBLE) upload_plan<double>(*H, P);
     else upload_plan<float>(*H, P);
@@ -2180,6 +2239,51 @@ bfe_status bfe_run(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
     else
       run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out,
                         field_every, field_is_vel, field_out, ti_start_out, nullptr, 0);
+  });
+}
+
+bfe_status bfe_set_field_elements(bfe_handle h, int64_t n_nodes, int32_t max_df,
+                                  const int32_t* node_dof, int64_t n_el, int32_t npe,
+                                  const int32_t* els) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (n_nodes < 1 || max_df < 1 || n_el < 1 || npe < 1 || npe > 8)
+      throw std::invalid_argument("field elements: bad sizes");
+    if (node_dof == nullptr || els == nullptr) throw std::invalid_argument("NULL field tables");
+    for (int64_t i = 0; i < n_nodes * max_df; ++i)
+      if (node_dof[i] < -1 || node_dof[i] >= h->n)
+        throw std::invalid_argument("field elements: DOF index out of range");
+    for (int64_t i = 0; i < n_el * npe; ++i)
+      if (els[i] < -1 || els[i] >= n_nodes)
+        throw std::invalid_argument("field elements: node index out of range");
+    BFE_HIP(hipSetDevice(h->device));
+    auto F = std::make_shared<FieldElements>();
+    F->n_el = n_el;
+    F->n_nodes = n_nodes;
+    F->npe = npe;
+    F->max_df = max_df;
+    F->els = upload(std::vector<int32_t>(els, els + n_el * npe));
+    F->node_dof = upload(std::vector<int32_t>(node_dof, node_dof + n_nodes * max_df));
+    if (h->prec == BFE_PREC_DOUBLE) F->m = upload(h->m_host);
+    else F->m = upload(cast_vec<float>(h->m_host));
+    h->field_elems = F;
+  });
+}
+
+bfe_status bfe_run_field_ke(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                            const double* frcs, int32_t frc_rows, const int32_t* hist_idx,
+                            int32_t q, double* hist_out, int32_t field_every, double* el_out,
+                            int32_t* ti_start_out) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (field_every <= 0) throw std::invalid_argument("field_every must be positive");
+    if (el_out == nullptr) throw std::invalid_argument("element field output is NULL");
+    if (h->prec == BFE_PREC_DOUBLE)
+      run_single<double>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, field_every,
+                         1, nullptr, ti_start_out, nullptr, 0, el_out);
+    else
+      run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, field_every,
+                        1, nullptr, ti_start_out, nullptr, 0, el_out);
   });
 }
 

@@ -25,7 +25,7 @@ from .assembly import fn_build_global_matrices_v5
 from .field_output import fn_get_field_output
 from .options import BRISTOLFE_DEFAULTS, ENTRY_POINT_DEFAULTS, set_default_fields
 from .ebe import element_operator
-from .solver import fn_explicit_dynamic_solver_v6, make_plan
+from .solver import field_frames, fn_explicit_dynamic_solver_v6, make_plan
 
 
 def fn_nds_and_dfs_to_gi(nds, dfs, gl_lookup):
@@ -118,11 +118,27 @@ def fn_BristolFE_v2(mod, matls, steps, fe_options=None, return_mats=False):
                 plans[key] = make_plan(K, C, M, dt, fe_options["solver_mode"],
                                        fe_options["solver_precision"], element_op=elem_op,
                                        operator=operator)
-            mon_dsps, fld, _, fld_time = fn_explicit_dynamic_solver_v6(
-                K, C, M, t, frc_gi, frcs, None, None, hist_gi,
-                fe_options["field_output_every_n_frames"], fe_options["use_gpu_if_available"],
-                fe_options["field_output_type"], fe_options["solver_mode"],
-                fe_options["solver_precision"], plan=plans[key])
+            frames = field_frames(t.size, fe_options["field_output_every_n_frames"])
+            el_fld = None
+            if (frames.size and fe_options["field_output_type"] == "KE"
+                    and fe_options.get("device_field_output", True)):
+                # 'KE' field reduced per element on the device (fn_get_field_output.m:64-79)
+                plan = plans[key]
+                if getattr(plan, "n_field_el", None) is None:
+                    plan.set_field_elements(mod["els"], gl_lookup)
+                F = np.asarray(frcs, np.float64)
+                F = F[None, :] if F.ndim == 1 else F
+                mon_dsps, el_fld, ti_start = plan.run_field_ke(
+                    t.size, np.asarray(frc_gi, np.int64) - 1, F if np.size(frc_gi) else np.zeros((1, t.size)),
+                    np.asarray(hist_gi, np.int64) - 1, int(fe_options["field_output_every_n_frames"]))
+                fld = None
+                fld_time = np.where(frames >= ti_start, t[frames - 1], 0.0)
+            else:
+                mon_dsps, fld, _, fld_time = fn_explicit_dynamic_solver_v6(
+                    K, C, M, t, frc_gi, frcs, None, None, hist_gi,
+                    fe_options["field_output_every_n_frames"], fe_options["use_gpu_if_available"],
+                    fe_options["field_output_type"], fe_options["solver_mode"],
+                    fe_options["solver_precision"], plan=plans[key])
             r["fld_time"] = fld_time
             if "nds" in mon:
                 valid_m = r["valid_mon_dsps"]
@@ -137,8 +153,11 @@ def fn_BristolFE_v2(mod, matls, steps, fe_options=None, return_mats=False):
                 r["dsp_gi"] = np.zeros(0, np.int64)
             r["frcs"] = np.zeros((0, 0))
             r["frc_gi"] = np.zeros(0, np.int64)
-            r["fld"] = (fn_get_field_output(fld, mod, mats, fe_options["field_output_type"])
-                        if fld.size else np.zeros((0, 0)))
+            if el_fld is not None:
+                r["fld"] = el_fld
+            else:
+                r["fld"] = (fn_get_field_output(fld, mod, mats, fe_options["field_output_type"])
+                            if fld.size else np.zeros((0, 0)))
             res.append(r)
     finally:
         for p in plans.values():

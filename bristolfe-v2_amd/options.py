@@ -26,6 +26,9 @@ BRISTOLFE_DEFAULTS = {
     # not in the reference: K u by the element-by-element operator when the
     # mesh has the template form ("auto"), always CSR ("csr"), or required ("ebe")
     "stiffness_operator": "auto",
+    # not in the reference: the 'KE' field output reduced per element on the
+    # device next to the march (True) or on the host from the frames (False)
+    "device_field_output": True,
 }
 
 SOLVER_MODES = {

@@ -140,6 +140,30 @@ bfe_status bfe_run(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
                    int32_t* ti_start_out);
 
 /*
+ * Per-element field output on the device (SURVEY next row: the reduction of
+ * code/internal/fn_get_field_output.m:64-79, called at
+ * code/fn_BristolFE_v2.m:145, moved next to the march).
+ *
+ * bfe_set_field_elements — the mesh tables of the reduction:
+ *   node_dof (n_nodes x max_df)  gl_lookup - 1: global DOF of (node, dof), -1 if absent
+ *   els (n_el x npe)             element nodes, 0-based, -1 where a row is shorter
+ * The lumped mass is the one given to bfe_create.
+ *
+ * bfe_run_field_ke — bfe_run with velocity frames every field_every steps
+ * reduced on the device to the 'KE' field: per element and frame
+ * sqrt(mean over the element's nodes of sum over the node's DOFs of
+ * |v^2 m| / 2). el_out is n_el x n_frames (column-major); the n x n_frames
+ * velocity frames stay on the device.
+ */
+bfe_status bfe_set_field_elements(bfe_handle h, int64_t n_nodes, int32_t max_df,
+                                  const int32_t* node_dof, int64_t n_el, int32_t npe,
+                                  const int32_t* els);
+bfe_status bfe_run_field_ke(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                            const double* frcs, int32_t frc_rows, const int32_t* hist_idx,
+                            int32_t q, double* hist_out, int32_t field_every, double* el_out,
+                            int32_t* ti_start_out);
+
+/*
  * bfe_run_batched — march n_src independent load cases on the same operator
  * in one launch sequence (sparse x multivector). Replaces the sequential
  * per-step loop of code/fn_BristolFE_v2.m:72-123 as driven by

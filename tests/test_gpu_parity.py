@@ -257,3 +257,27 @@ def test_graph_captured_march_matches_eager(operator):
             np.testing.assert_array_equal(h2, ref)
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_device_ke_field_matches_host_reduction(prec):
+    """bfe_run_field_ke: the 'KE' field reduced per element on the device
+    against the oracle's reduction of the oracle's velocity frames (coupled
+    model: CPE3, AC2D3 and 2-node ASI2D2 rows)."""
+    print("case: device KE field", prec, flush=True)
+    from oracle import post as op
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+    K, C, M, gl, t, F, fi, hi = _setup(mod, matls, opts, 400, 5e6, src=[1.5e-3, 2.0e-3], dof=4)
+    plan = PKG.make_plan(K, C, M, t[1] - t[0], solver_precision=prec)
+    try:
+        plan.set_field_elements(mod["els"], gl)
+        h, el, ti = plan.run_field_ke(t.size, fi - 1, F, hi - 1, 40)
+    finally:
+        plan.close()
+    r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, 40, "KE")
+    ke = op.field_output_ke(rf, mod["els"], M.diagonal(), gl)
+    assert ti == rti and el.shape == ke.shape and np.linalg.norm(ke) > 0
+    tol = TOL if prec == "double" else 1e-3
+    assert rel_l2(h, r) < tol
+    assert rel_l2(el, ke) < tol
