@@ -590,6 +590,19 @@ struct ClassOffsets {
   int32_t off[NC];
 };
 
+// Where slot k's element keeps its scale in the anchor table (n_rows x
+// n_nodes, row = the element's template, column = its canonical first
+// vertex): entry v + base[k] on a regular node v, base[k] = row * n_nodes +
+// the node offset of the first vertex; len = the table's length. Only
+// patterns of few slots take it: the per-slot kernel arguments compete with
+// the template rows for scalar registers (24 slots spill).
+constexpr int kAnchorMaxSlots = 8;
+template <int NS>
+struct AnchorSlots {
+  int64_t base[NS];
+  int64_t len;
+};
+
 // Element-by-element step on a mesh pattern (at most 128 VGPRs, so that four
 // wavefronts share a SIMD: the kernel is bound by load latency more than by
 // its FP64 FMAs). Node v's first DOF is r0 (dof0[v], or v*DPN when dof0 is
@@ -597,17 +610,21 @@ struct ClassOffsets {
 // registers; on a regular node the first DOF of neighbour j is r0 + off[j]
 // (irregular nodes, flagged in bit 31 of pmask, read it from the class-major
 // neighbour table nbr). pmask bit k marks an occupied slot, smask bit k a slot
-// whose element scale differs from 1: only those read the slot table and the
-// scale. DIRECT groups (most slots scaled) read instead a slot-major table of
-// per-slot scales, 0 for an empty slot, with no table -> element indirection.
+// whose element scale differs from 1: only those read a scale, through slot
+// table -> element -> scale. ANCHOR groups (most slots scaled, e.g. every
+// triangle, whose scale carries J_rep / J_e) read a scale for every occupied
+// slot, on a regular node from the anchor table, where each element's scale
+// sits at (its template, its canonical first vertex): a wave's reads are
+// coalesced and the element's other vertices find the line in cache.
+// Irregular nodes follow the table -> element chain.
 // Template rows (canonical vertex order) come through the scalar cache.
-template <typename T, bool DAMPED, typename PT, int DPN, bool DIRECT>
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
     int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
-    ClassOffsets<PT::NC> co, const uint32_t* __restrict__ pmask,
+    ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an, const uint32_t* __restrict__ pmask,
     const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
     const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
-    const T* __restrict__ slot_scale,
+    const T* __restrict__ anchor,
     const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
     const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
     T* __restrict__ u, BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F,
@@ -617,11 +634,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t v = lb * kBlock + threadIdx.x;
   if (v >= n_nodes) return;
-  const uint32_t pm = pmask[v], sm = smask[v];
+  constexpr uint32_t kAll = (1u << NS) - 1u;
+  const uint32_t pm = pmask[v];
+  const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
   const int64_t ib = pm >> 31;  // 1: irregular node
   const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
-  constexpr uint32_t kAll = (1u << NS) - 1u;
-  if (!DIRECT && __all((pm & kAll) == kAll && sm == 0u)) {
+  if (!ANCHOR && __all((pm & kAll) == kAll && sm == 0u)) {
     // every lane's elements are all present with scale 1 (the interior): K u
     // is the fixed stencil sum_j B_j u_j, B_j the sum of the template blocks
     // of neighbour class j (NC DPN x DPN blocks instead of NS template rows)
@@ -672,11 +690,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     // predicates as integer lane values (0 or 1) and bit masks: kept per
     // lane in VGPRs, where 24 slots' worth of boolean lane masks would
     // exhaust the scalar registers
+    const uint32_t sb = (sm >> k) & 1u;
     T s;
-    if constexpr (DIRECT) {
-      s = slot_scale[static_cast<int64_t>(k) * n_nodes + v];  // 0 for an empty slot
+    if constexpr (ANCHOR) {
+      // occupied slot on a regular node: its own entry; else any entry in
+      // range (the slot's product is masked out)
+      int64_t a = v + an.base[k];
+      a = (a < 0 || a >= an.len) ? 0 : a;
+      const uint32_t rb = sb & static_cast<uint32_t>(1 - ib);
+      T sa = anchor[a * rb];
+      const uint32_t xb = sb & static_cast<uint32_t>(ib);
+      const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * xb];
+      sa = pick_bits(el_scale[static_cast<int64_t>(e) * xb], sa, xb);
+      s = pick_bits(sa, T(1), sb);
     } else {
-      const uint32_t sb = (sm >> k) & 1u;
       const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * sb];
       s = pick_bits(el_scale[static_cast<int64_t>(e) * sb], T(1), sb);
     }
@@ -690,14 +717,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       for (int d = 0; d < DPN; ++d)
 #pragma unroll
         for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
-    if constexpr (DIRECT) {
+    const uint32_t ob = (pm >> k) & 1u;
 #pragma unroll
-      for (int c = 0; c < DPN; ++c) acc[c] += s * y[c];
-    } else {
-      const uint32_t ob = (pm >> k) & 1u;
-#pragma unroll
-      for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
-    }
+    for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
   }
   ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
                              q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
@@ -890,14 +912,17 @@ struct EbeGroup {
   int32_t dpn = 0, npe = 0, n_slot = 0, n_tmpl = 0, n_cls = 0;
   int32_t pattern = 0;       // mesh pattern of k_step_ebe_pat, 0: none
   bool uniform = false;      // node j's first DOF is j*dpn (no DOF table read)
-  bool direct = false;       // per-slot scale table instead of table -> element -> scale
+  bool smask_all = false;    // most occupied slots scaled (anchor table, no smask)
+  int32_t coef_node = 0;     // alpha/beta per group node (alpha_n, beta_n)
   std::vector<int64_t> doff; // first-DOF offset of each neighbour class on regular nodes
-  DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil, slot_scale;
+  std::vector<int32_t> anc_noff, anc_row;  // anchor table column offset and row per slot
+  DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil, anchor;
+  DevBuf alpha_n, beta_n;    // per node (coef_node)
   DevBuf iptr, irow, ik;     // interface rows (group-local rows) per workgroup
   int64_t bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&dof0, &adj, &nodes, &scale, &srows, &nbr, &srows_c, &pmask, &smask,
-                            &stencil, &slot_scale, &iptr, &irow, &ik})
+                            &stencil, &anchor, &alpha_n, &beta_n, &iptr, &irow, &ik})
       s += static_cast<int64_t>(b->bytes);
     return s;
   }
@@ -909,8 +934,7 @@ struct EbeOp {
   // global row -> (group, group-local row); group -1 when no group owns it
   std::vector<int8_t> row_group;
   std::vector<int32_t> row_local;
-  int32_t coef_per_node = 0;  // one group, uniform numbering, alpha/beta equal per node
-  DevBuf alpha, beta;         // per DOF, or per node (coef_per_node)
+  DevBuf alpha, beta;         // per DOF, for groups without per-node coefficients
   double check_err = 0.0;     // largest relative difference to the CSR step at set-up
   int64_t n_el() const {
     int64_t s = 0;
@@ -921,6 +945,15 @@ struct EbeOp {
     int64_t s = static_cast<int64_t>(alpha.bytes + beta.bytes);
     for (const auto& g : groups) s += g.bytes();
     return s;
+  }
+  // alpha/beta a group's kernel reads: per group node or per global DOF
+  template <typename T>
+  const T* alpha_of(const EbeGroup& g) const {
+    return g.coef_node ? g.alpha_n.as<T>() : alpha.as<T>();
+  }
+  template <typename T>
+  const T* beta_of(const EbeGroup& g) const {
+    return g.coef_node ? g.beta_n.as<T>() : beta.as<T>();
   }
 };
 
@@ -1287,8 +1320,8 @@ void launch_ebe_generic_g(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& G
   hipLaunchKernelGGL((k_step_ebe<T, D, NPE, DPN, G>), dim3(nblk), dim3(kBlock), 0, H.stream,
                      Gr.n_nodes, nblk, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), Gr.n_slot,
                      Gr.adj.as<int32_t>(), Gr.nodes.as<int4>(), Gr.scale.as<T>(),
-                     Gr.srows.as<T>(), E.alpha.as<T>(), D ? E.beta.as<T>() : nullptr, u1, u2, u,
-                     L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, E.coef_per_node);
+                     Gr.srows.as<T>(), E.alpha_of<T>(Gr), D ? E.beta_of<T>(Gr) : nullptr, u1, u2,
+                     u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR, Gr.coef_node);
 }
 
 template <typename T, bool D, int NPE, int DPN>
@@ -1310,25 +1343,40 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int
                     int32_t fld_is_vel, T dt, const IfaceRows<T>& IR) {
   ClassOffsets<PT::NC> co;
   for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(Gr.doff[j]);
-  if (Gr.direct)
-    hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, true>), dim3(nblk), dim3(kBlock), 0,
+  AnchorSlots<PT::NS> an{};
+  const bool anchored = Gr.anchor.bytes > 0;  // only on groups whose slots are mostly scaled
+  if (anchored) {
+    for (int k = 0; k < PT::NS; ++k)
+      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
+    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+  }
+  const uint32_t* sm = Gr.smask.as<uint32_t>();
+  if constexpr (PT::NS <= kAnchorMaxSlots) {
+    if (anchored) {
+      hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, true>), dim3(nblk), dim3(kBlock), 0,
                        H.stream, Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(),
-                       co, Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
-                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.slot_scale.as<T>(),
-                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha.as<T>(),
-                       D ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
-                       fp, fld_is_vel, dt, IR, E.coef_per_node);
-  else
-    hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, false>), dim3(nblk), dim3(kBlock), 0,
+                       co, an, Gr.pmask.as<uint32_t>(), sm, Gr.nbr.as<int32_t>(),
+                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.anchor.as<T>(),
+                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha_of<T>(Gr),
+                       D ? E.beta_of<T>(Gr) : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                       fp, fld_is_vel, dt, IR, Gr.coef_node);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, false>), dim3(nblk), dim3(kBlock), 0,
                        H.stream, Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(),
-                       co, Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
-                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), Gr.slot_scale.as<T>(),
-                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha.as<T>(),
-                       D ? E.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
-                       fp, fld_is_vel, dt, IR, E.coef_per_node);
+                       co, an, Gr.pmask.as<uint32_t>(), sm, Gr.nbr.as<int32_t>(),
+                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), static_cast<const T*>(nullptr),
+                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha_of<T>(Gr),
+                       D ? E.beta_of<T>(Gr) : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
+                       fp, fld_is_vel, dt, IR, Gr.coef_node);
 }
 
 // Whether a kernel exists for the group's shape (generic) or pattern.
+bool pattern_has_anchor(int pattern) {
+  return (pattern == PatTriRect::ID && PatTriRect::NS <= kAnchorMaxSlots) ||
+         (pattern == PatTriIso::ID && PatTriIso::NS <= kAnchorMaxSlots);
+}
 bool has_generic_kernel(int npe, int dpn) {
   return (npe == 3 && dpn >= 1 && dpn <= 3) || (npe == 4 && dpn == 3);
 }
@@ -2012,6 +2060,32 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
     Gr.n_tmpl = in.n_tmpl;
     Gr.n_cls = Lay.n_cls;
     Gr.uniform = in.node_dof == nullptr;
+    // alpha and beta per group node when each node's DOFs share them
+    {
+      bool per_node = true;
+      for (int64_t j = 0; j < nn && per_node; ++j)
+        for (int c = 1; c < dpn; ++c)
+          if (h->alpha[dof0[j] + c] != h->alpha[dof0[j]] ||
+              (h->damped && h->beta_host[dof0[j] + c] != h->beta_host[dof0[j]])) {
+            per_node = false;
+            break;
+          }
+      if (per_node) {
+        std::vector<double> an(nn), bn(h->damped ? nn : 0);
+        for (int64_t j = 0; j < nn; ++j) {
+          an[j] = h->alpha[dof0[j]];
+          if (h->damped) bn[j] = h->beta_host[dof0[j]];
+        }
+        Gr.coef_node = 1;
+        if (h->prec == BFE_PREC_DOUBLE) {
+          Gr.alpha_n = upload(an);
+          if (h->damped) Gr.beta_n = upload(bn);
+        } else {
+          Gr.alpha_n = upload(cast_vec<float>(an));
+          if (h->damped) Gr.beta_n = upload(cast_vec<float>(bn));
+        }
+      }
+    }
     Gr.pattern = match_pattern(Lay);
     if (Gr.pattern != 0 && !has_pattern_kernel(Gr.pattern, dpn)) Gr.pattern = 0;
     // element records hold each vertex's first DOF
@@ -2032,10 +2106,14 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
         if (all) ref = v;
       }
       if (ref < 0) throw std::invalid_argument("element group: no interior node");
-      for (int32_t j = 0; j < nc; ++j)
-        Gr.doff[j] = static_cast<int64_t>(dof0[Lay.nbr[static_cast<size_t>(j) * nn + ref]]) -
-                     dof0[ref];
-      // neighbour table as first DOFs; irregular flag (bit 31) in DOF terms
+      std::vector<int64_t> noff(nc, 0);  // node-index offsets of the classes
+      for (int32_t j = 0; j < nc; ++j) {
+        const int32_t w = Lay.nbr[static_cast<size_t>(j) * nn + ref];
+        Gr.doff[j] = static_cast<int64_t>(dof0[w]) - dof0[ref];
+        noff[j] = static_cast<int64_t>(w) - ref;
+      }
+      // neighbour table as first DOFs; a node is irregular (bit 31) when a
+      // neighbour is not at its class offset, in DOFs or in node numbers
       std::vector<int32_t> nbrdof(Lay.nbr.size());
       std::vector<uint32_t> pm(Lay.present);
       for (int64_t v = 0; v < nn; ++v) {
@@ -2043,7 +2121,7 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
         for (int32_t j = 0; j < nc; ++j) {
           const int32_t w = Lay.nbr[static_cast<size_t>(j) * nn + v];
           nbrdof[static_cast<size_t>(j) * nn + v] = w < 0 ? -1 : dof0[w];
-          if (w >= 0 && dof0[w] != dof0[v] + Gr.doff[j]) irregular = true;
+          if (w >= 0 && (dof0[w] != dof0[v] + Gr.doff[j] || w != v + noff[j])) irregular = true;
         }
         pm[v] = (pm[v] & 0x7fffffffu) | (irregular ? 0x80000000u : 0u);
       }
@@ -2058,22 +2136,63 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
       Gr.pmask = upload(pm);
       Gr.smask = upload(sm);
       // most occupied slots scaled (e.g. every triangle, whose scale carries
-      // J_rep / J_e): a slot-major scale table beats the table -> element chain
+      // J_rep / J_e): every occupied slot reads its scale from an anchor table
       int64_t occupied = 0, scaled = 0;
       for (int64_t v = 0; v < nn; ++v) {
         occupied += __builtin_popcount(pm[v] & 0x7fffffffu);
         scaled += __builtin_popcount(sm[v]);
       }
-      Gr.direct = 2 * scaled > occupied;
-      if (Gr.direct) {
-        std::vector<double> ss(static_cast<size_t>(Lay.n_slot) * nn, 0.0);
-        for (int32_t k = 0; k < Lay.n_slot; ++k)
-          for (int64_t v = 0; v < nn; ++v) {
+      Gr.smask_all = 2 * scaled > occupied;
+      // anchor table: element e's scale at (row of its template, node of its
+      // canonical first vertex), if every slot view of e agrees on that place
+      // and no two elements share one
+      {
+        const int32_t acls_self = Lay.self_cls;
+        std::vector<int32_t> trow(in.n_tmpl, -1);
+        int32_t n_rows = 0;
+        std::vector<int32_t> srow(Lay.n_slot, -1);
+        std::vector<int64_t> el_pos(static_cast<size_t>(ne), -1);
+        bool ok = Gr.smask_all && pattern_has_anchor(Gr.pattern);
+        for (int32_t k = 0; k < Lay.n_slot && ok; ++k) {
+          const int32_t ac = Lay.cls[static_cast<size_t>(k) * npe];
+          for (int64_t v = 0; v < nn && ok; ++v) {
             const int32_t e = Lay.adj[static_cast<size_t>(k) * nn + v];
-            if (e >= 0) ss[static_cast<size_t>(k) * nn + v] = in.el_scale[e];
+            if (e < 0) continue;
+            const int32_t tau = in.el_tmpl[e];
+            if (trow[tau] < 0) trow[tau] = n_rows++;
+            if (srow[k] < 0) srow[k] = trow[tau];
+            const int64_t a = ac == acls_self ? v : Lay.nbr[static_cast<size_t>(ac) * nn + v];
+            if (srow[k] != trow[tau] || a < 0) {
+              ok = false;
+              break;
+            }
+            const int64_t pos = static_cast<int64_t>(trow[tau]) * nn + a;
+            if (el_pos[e] < 0) el_pos[e] = pos;
+            else if (el_pos[e] != pos) ok = false;
           }
-        if (h->prec == BFE_PREC_DOUBLE) Gr.slot_scale = upload(ss);
-        else Gr.slot_scale = upload(cast_vec<float>(ss));
+        }
+        std::vector<double> anc;
+        if (ok) {
+          anc.assign(static_cast<size_t>(n_rows) * nn, 0.0);
+          std::vector<int32_t> taken(static_cast<size_t>(n_rows) * nn, -1);
+          for (int64_t e = 0; e < ne && ok; ++e) {
+            if (el_pos[e] < 0) continue;
+            if (taken[el_pos[e]] >= 0) ok = false;
+            taken[el_pos[e]] = static_cast<int32_t>(e);
+            anc[el_pos[e]] = in.el_scale[e];
+          }
+        }
+        if (ok) {
+          Gr.anc_noff.assign(Lay.n_slot, 0);
+          Gr.anc_row.assign(Lay.n_slot, 0);
+          for (int32_t k = 0; k < Lay.n_slot; ++k) {
+            const int32_t ac = Lay.cls[static_cast<size_t>(k) * npe];
+            Gr.anc_noff[k] = static_cast<int32_t>(ac == acls_self ? 0 : noff[ac]);
+            Gr.anc_row[k] = std::max(srow[k], 0);
+          }
+          if (h->prec == BFE_PREC_DOUBLE) Gr.anchor = upload(anc);
+          else Gr.anchor = upload(cast_vec<float>(anc));
+        }
       }
       // summed stencil: B_j = sum over (slot, vertex) of class j
       stencil.assign(static_cast<size_t>(nc) * dpn * dpn, 0.0);
@@ -2124,42 +2243,25 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
     Gr.irow = upload(g.row);
     Gr.ik = upload(g.a);
   }
-  // alpha and beta per node when one uniform group's nodes share them
-  const std::vector<double>& al = h->alpha;
-  bool per_node = n_groups == 1 && E->groups[0].uniform;
-  const int32_t dpn0 = E->groups[0].dpn;
-  const int64_t nn0 = E->groups[0].n_nodes;
-  for (int64_t v = 0; v < nn0 && per_node; ++v)
-    for (int c = 1; c < dpn0; ++c)
-      if (al[v * dpn0 + c] != al[v * dpn0] ||
-          (h->damped && h->beta_host[v * dpn0 + c] != h->beta_host[v * dpn0])) {
-        per_node = false;
-        break;
-      }
-  E->coef_per_node = per_node ? 1 : 0;
-  std::vector<double> alpha_e, beta_e;
-  if (per_node) {
-    alpha_e.resize(nn0);
-    for (int64_t v = 0; v < nn0; ++v) alpha_e[v] = al[v * dpn0];
-    if (h->damped) {
-      beta_e.resize(nn0);
-      for (int64_t v = 0; v < nn0; ++v) beta_e[v] = h->beta_host[v * dpn0];
-    }
-  } else {
-    alpha_e = al;
-    if (h->damped) beta_e = h->beta_host;
+  // per-DOF alpha and beta for groups whose nodes' DOFs do not share them
+  bool per_dof = false, any_pattern = false;
+  for (const auto& g : E->groups) {
+    per_dof = per_dof || !g.coef_node;
+    any_pattern = any_pattern || g.pattern != 0;
   }
-  bool any_pattern = false;
-  for (const auto& g : E->groups) any_pattern = any_pattern || g.pattern != 0;
   double err;
   if (h->prec == BFE_PREC_DOUBLE) {
-    E->alpha = upload(alpha_e);
-    if (h->damped) E->beta = upload(beta_e);
+    if (per_dof) {
+      E->alpha = upload(h->alpha);
+      if (h->damped) E->beta = upload(h->beta_host);
+    }
     err = compare_ebe_with_csr<double>(*h, *E, true);
     if (any_pattern) err = std::max(err, compare_ebe_with_csr<double>(*h, *E, false));
   } else {
-    E->alpha = upload(cast_vec<float>(alpha_e));
-    if (h->damped) E->beta = upload(cast_vec<float>(beta_e));
+    if (per_dof) {
+      E->alpha = upload(cast_vec<float>(h->alpha));
+      if (h->damped) E->beta = upload(cast_vec<float>(h->beta_host));
+    }
     err = compare_ebe_with_csr<float>(*h, *E, true);
     if (any_pattern) err = std::max(err, compare_ebe_with_csr<float>(*h, *E, false));
   }
