@@ -434,9 +434,6 @@ def main():
         out["config"]["operator"] = (f"element-by-element: {op.n_el} elements in "
                                      f"{info['ebe_groups']} group(s), {info['ebe_templates']} "
                                      f"templates, {info['ebe_slots']} slots")
-        out["roofline"]["stored_bytes_per_dof_step"] = round(
-            ebe.ebe_stored_bytes_per_dof(op, n, info["ebe_slots"], info["damped"],
-                                         info["ebe_classes"] if pattern else 0), 2)
         # the CSR operator of the same plan, one march, for comparison
         plan.set_tuning(3, 0)
         plan.set_kernel_timing(args.timing_every)

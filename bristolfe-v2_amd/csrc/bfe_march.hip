@@ -607,54 +607,58 @@ struct AnchorSlots {
 // wavefronts share a SIMD: the kernel is bound by load latency more than by
 // its FP64 FMAs). Node v's first DOF is r0 (dof0[v], or v*DPN when dof0 is
 // NULL). The NC neighbours' u1 values are gathered once and kept in
-// registers; on a regular node the first DOF of neighbour j is r0 + off[j]
-// (irregular nodes, flagged in bit 31 of pmask, read it from the class-major
-// neighbour table nbr). pmask bit k marks an occupied slot, smask bit k a slot
-// whose element scale differs from 1: only those read a scale, through slot
-// table -> element -> scale. ANCHOR groups (most slots scaled, e.g. every
-// triangle, whose scale carries J_rep / J_e) read a scale for every occupied
-// slot, on a regular node from the anchor table, where each element's scale
-// sits at (its template, its canonical first vertex): a wave's reads are
-// coalesced and the element's other vertices find the line in cache.
-// Irregular nodes follow the table -> element chain.
-// Template rows (canonical vertex order) come through the scalar cache.
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
-    int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
-    ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an, const uint32_t* __restrict__ pmask,
-    const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
-    const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
-    const T* __restrict__ anchor,
-    const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
-    const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
-    T* __restrict__ u, BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F,
-    int32_t F_rows, int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
-    int32_t fld_is_vel, T dt, IfaceRows<T> I, int32_t coef_per_node) {
+// registers; on a regular node the first DOF of neighbour j is r0 + off[j].
+// Irregular nodes (bit 31 of pmask) read it from the class-major neighbour
+// table nbr; in ANCHOR groups a wave that holds one takes the general path
+// (REG = false), the others a path without those reads. pmask bit k marks an occupied slot,
+// smask bit k a slot whose element scale differs from 1: only those read a
+// scale, through slot table -> element -> scale. ANCHOR groups (most slots
+// scaled, e.g. every triangle, whose scale carries J_rep / J_e) read a scale
+// for every occupied slot, on a regular node from the anchor table, where
+// each element's scale sits at (its template, its canonical first vertex): a
+// wave's reads are coalesced and the element's other vertices find the line
+// in cache. Template rows (canonical vertex order) come through the scalar
+// cache.
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool REG>
+__device__ __forceinline__ void ebe_pat_node(
+    int64_t v, int64_t lb, int64_t n_nodes, int64_t n, int64_t r0, uint32_t pm, uint32_t sm,
+    const ClassOffsets<PT::NC>& co, const AnchorSlots<PT::NS>& an,
+    const int32_t* __restrict__ nbr, const int32_t* __restrict__ adj,
+    const T* __restrict__ el_scale, const T* __restrict__ anchor, const T* __restrict__ srows,
+    const T* __restrict__ stencil, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, T* __restrict__ fld_prev, int32_t fld_is_vel, T dt,
+    const IfaceRows<T>& I, int32_t coef_per_node) {
   constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
-  const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t v = lb * kBlock + threadIdx.x;
-  if (v >= n_nodes) return;
   constexpr uint32_t kAll = (1u << NS) - 1u;
-  const uint32_t pm = pmask[v];
-  const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
-  const int64_t ib = pm >> 31;  // 1: irregular node
-  const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
+  const int64_t ib = REG ? 0 : static_cast<int64_t>(pm >> 31);  // 1: irregular node
+  // first DOF of neighbour j (an offset out of range belongs to an absent
+  // neighbour, used only by empty slots)
+  auto nb = [&](int j) -> int64_t {
+    if (j == PT::SELF) return r0;
+    const int64_t wr = r0 + co.off[j];
+    const int64_t wreg = (wr < 0 || wr > n - DPN) ? r0 : wr;
+    if constexpr (REG) {
+      return wreg;
+    } else {
+      // lanes that do not need a table entry read entry 0 instead, one
+      // cached line for the wave (no divergent branch around the load)
+      const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
+      const int64_t wi = wj < 0 ? r0 : wj;
+      return ib * wi + (1 - ib) * wreg;
+    }
+  };
+  T acc[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
   if (!ANCHOR && __all((pm & kAll) == kAll && sm == 0u)) {
     // every lane's elements are all present with scale 1 (the interior): K u
     // is the fixed stencil sum_j B_j u_j, B_j the sum of the template blocks
     // of neighbour class j (NC DPN x DPN blocks instead of NS template rows)
-    T acc[DPN];
-#pragma unroll
-    for (int c = 0; c < DPN; ++c) acc[c] = T(0);
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
-      int64_t w = r0;
-      if (j != PT::SELF) {
-        const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
-        const int64_t wr = r0 + co.off[j];
-        const int64_t wi = wj < 0 ? r0 : wj;
-        w = ib * wi + (1 - ib) * (wr < 0 || wr > n - DPN ? r0 : wr);
-      }
+      const int64_t w = nb(j);
 #pragma unroll
       for (int d = 0; d < DPN; ++d) {
         const T xd = u1[w + d];
@@ -666,25 +670,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
     return;
   }
-  // Loads are issued on every lane without branching (a divergent branch
-  // around a load serialises the wave on its latency); lanes that do not
-  // need a table entry read entry 0 instead, one cached line for the wave.
   T x[NC][DPN];
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
-    int64_t w = r0;
-    if (j != PT::SELF) {
-      const int32_t wj = nbr[(static_cast<int64_t>(j) * n_nodes + v) * ib];
-      const int64_t wr = r0 + co.off[j];  // out of range: absent, used only by empty slots
-      const int64_t wi = wj < 0 ? r0 : wj;
-      w = ib * wi + (1 - ib) * (wr < 0 || wr > n - DPN ? r0 : wr);
-    }
+    const int64_t w = nb(j);
 #pragma unroll
     for (int d = 0; d < DPN; ++d) x[j][d] = u1[w + d];
   }
-  T acc[DPN];
-#pragma unroll
-  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     // predicates as integer lane values (0 or 1) and bit masks: kept per
@@ -699,9 +691,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       a = (a < 0 || a >= an.len) ? 0 : a;
       const uint32_t rb = sb & static_cast<uint32_t>(1 - ib);
       T sa = anchor[a * rb];
-      const uint32_t xb = sb & static_cast<uint32_t>(ib);
-      const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * xb];
-      sa = pick_bits(el_scale[static_cast<int64_t>(e) * xb], sa, xb);
+      if constexpr (!REG) {
+        const uint32_t xb = sb & static_cast<uint32_t>(ib);
+        const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * xb];
+        sa = pick_bits(el_scale[static_cast<int64_t>(e) * xb], sa, xb);
+      }
       s = pick_bits(sa, T(1), sb);
     } else {
       const int32_t e = adj[(static_cast<int64_t>(k) * n_nodes + v) * sb];
@@ -723,6 +717,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   }
   ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
                              q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+}
+
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
+    int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
+    ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an, const uint32_t* __restrict__ pmask,
+    const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
+    const T* __restrict__ anchor,
+    const T* __restrict__ srows, const T* __restrict__ stencil, const T* __restrict__ alpha,
+    const T* __restrict__ beta, const T* __restrict__ u1, const T* __restrict__ u2,
+    T* __restrict__ u, BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F,
+    int32_t F_rows, int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
+    int32_t fld_is_vel, T dt, IfaceRows<T> I, int32_t coef_per_node) {
+  constexpr uint32_t kAll = (1u << PT::NS) - 1u;
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  const int64_t v = lb * kBlock + threadIdx.x;
+  if (v >= n_nodes) return;
+  const uint32_t pm = pmask[v];
+  const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
+  const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
+  // (the split pays on the anchored triangle groups; on the 24-slot tet
+  // pattern the second copy costs more scalar registers than it saves)
+  if (!ANCHOR || __any(static_cast<int>(pm >> 31)))
+    ebe_pat_node<T, DAMPED, PT, DPN, ANCHOR, false>(
+        v, lb, n_nodes, n, r0, pm, sm, co, an, nbr, adj, el_scale, anchor, srows, stencil, alpha,
+        beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, fld_prev, fld_is_vel, dt, I,
+        coef_per_node);
+  else
+    ebe_pat_node<T, DAMPED, PT, DPN, ANCHOR, true>(
+        v, lb, n_nodes, n, r0, pm, sm, co, an, nbr, adj, el_scale, anchor, srows, stencil, alpha,
+        beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, fld_prev, fld_is_vel, dt, I,
+        coef_per_node);
 }
 
 // Pattern id of a layout's class table, 0 if none matches.

@@ -210,19 +210,3 @@ def ebe_bytes_per_dof(op, n: int, damped: bool) -> float:
     streams (alpha, plus beta when damped)."""
     el = sum(g.n_el * (4 * g.el_nodes.shape[1] + 1 + 8) for g in op.groups)
     return el / n + 24 + 8 * (2 if damped else 1)
-
-
-def ebe_stored_bytes_per_dof(op, n: int, n_slot: int, damped: bool,
-                             n_classes: int = 0) -> float:
-    """Bytes per DOF-step this implementation streams if every record is read
-    once: the slot-major node-to-element table (4 n_slot per node), and either
-    (mesh-pattern kernel, n_classes > 0) the neighbour table (4 per neighbour
-    class other than the node itself) and the element scale (8), or (generic
-    kernel) the padded node quads (16) and the scale (8) per element; plus the
-    vectors and coefficient streams."""
-    if n_classes > 0:
-        per_node, per_el = 4.0 * n_slot + 4.0 * (n_classes - 1), 8.0
-    else:
-        per_node, per_el = 4.0 * n_slot, 24.0
-    nodes = sum(g.n_nodes for g in op.groups)
-    return (per_node * nodes + per_el * op.n_el) / n + 24 + 8 * (2 if damped else 1)

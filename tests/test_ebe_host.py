@@ -121,4 +121,3 @@ def test_bytes_per_dof_formulae():
     n = K.shape[0]
     b = _ebe().ebe_bytes_per_dof(op, n, True)
     assert np.isclose(b, op.n_el / n * 25 + 24 + 16)
-    assert _ebe().ebe_stored_bytes_per_dof(op, n, 24, True) > b
