@@ -35,7 +35,8 @@ EXPORTS = (
     "bfe_set_kernel_timing",
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
-    "bfe_assemble_fill", "bfe_assemble_last_error",
+    "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
+    "bfe_assemble_device_fill", "bfe_assemble_device_free",
 )
 
 
@@ -144,6 +145,16 @@ def load() -> C.CDLL:
     lib.bfe_assemble_fill.restype = C.c_int
     lib.bfe_assemble_last_error.argtypes = []
     lib.bfe_assemble_last_error.restype = C.c_char_p
+    lib.bfe_assemble_device.argtypes = [C.c_int32, C.c_int64, C.c_int32, _f64p, C.c_int64, C.c_int32,
+                                        _i32p, _i32p, _i32p, _f64p, C.c_int32, _f64p, _f64p,
+                                        C.c_int32, C.c_double, C.c_double, C.c_double, _i64p, _i64p,
+                                        _i64p, _i32p, _p(C.c_void_p)]
+    lib.bfe_assemble_device.restype = C.c_int
+    lib.bfe_assemble_device_fill.argtypes = [C.c_void_p, _i64p, _i32p, _f64p, _i64p, _i32p, _f64p,
+                                             _f64p, _i64p]
+    lib.bfe_assemble_device_fill.restype = C.c_int
+    lib.bfe_assemble_device_free.argtypes = [C.c_void_p]
+    lib.bfe_assemble_device_free.restype = C.c_int
     _lib = lib
     return lib
 
@@ -405,3 +416,38 @@ def assemble_native(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mas
     if st != BFE_OK:
         raise BfeError(st, lib.bfe_assemble_last_error().decode())
     return rowptr, col, kval, m, c, gl
+
+
+def assemble_device(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mask, p_law,
+                    max_damping, msr, device=0):
+    """Calls bfe_assemble_device/fill (GPU). 0-based element nodes and
+    materials (ignored for ASI2D2). Returns (k_rowptr, k_col, k_val, c_rowptr,
+    c_col, c_val, m, gl_lookup)."""
+    lib = load()
+    nds = np.ascontiguousarray(nds, np.float64)
+    els0 = np.ascontiguousarray(els0, np.int32)
+    el_type = np.ascontiguousarray(el_type, np.int32)
+    el_mat0 = np.ascontiguousarray(el_mat0, np.int32)
+    el_abs = np.ascontiguousarray(el_abs, np.float64)
+    mat_D = np.ascontiguousarray(mat_D, np.float64)
+    mat_rho = np.ascontiguousarray(mat_rho, np.float64)
+    n_dof, zk, zc, max_df, work = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32(), C.c_void_p()
+    st = lib.bfe_assemble_device(
+        int(device), nds.shape[0], nds.shape[1], ptr(nds, C.c_double), els0.shape[0],
+        els0.shape[1], ptr(els0, C.c_int32), ptr(el_type, C.c_int32), ptr(el_mat0, C.c_int32),
+        ptr(el_abs, C.c_double), mat_rho.size, ptr(mat_D, C.c_double), ptr(mat_rho, C.c_double),
+        int(dof_mask), float(p_law), float(max_damping), float(msr), C.byref(n_dof), C.byref(zk),
+        C.byref(zc), C.byref(max_df), C.byref(work))
+    if st != BFE_OK:
+        raise BfeError(st, lib.bfe_assemble_last_error().decode())
+    n = n_dof.value
+    out = (np.empty(n + 1, np.int64), np.empty(zk.value, np.int32), np.empty(zk.value, np.float64),
+           np.empty(n + 1, np.int64), np.empty(zc.value, np.int32), np.empty(zc.value, np.float64),
+           np.empty(n, np.float64), np.empty((nds.shape[0], max_df.value), np.int64))
+    st = lib.bfe_assemble_device_fill(
+        work, ptr(out[0], C.c_int64), ptr(out[1], C.c_int32), ptr(out[2], C.c_double),
+        ptr(out[3], C.c_int64), ptr(out[4], C.c_int32), ptr(out[5], C.c_double),
+        ptr(out[6], C.c_double), ptr(out[7], C.c_int64))
+    if st != BFE_OK:
+        raise BfeError(st, lib.bfe_assemble_last_error().decode())
+    return out

@@ -41,12 +41,16 @@ DEFAULTS = {
     "max_stiffness_reduction": 0.01,
     "interface_damping_factor": 0,
     # "numpy": vectorised coordinate assembly (any element type);
-    # "native": libbfe_hip's row-by-row assembler (CPE3/AC2D3/C3D4, no
-    # interface elements); "auto": native for more than 2M such elements.
+    # "native": libbfe_hip's row-by-row host assembler (CPE3/AC2D3/C3D4, no
+    # interface elements); "device": the same on the GPU, with ASI2D2
+    # interface elements too; "auto": the GPU for more than 200k elements
+    # when one is present, else native for more than 2M elements without
+    # interface elements, else numpy.
     "assembly_backend": "auto",
 }
 
 _NATIVE_TYPES = {"CPE3": 0, "AC2D3": 1, "C3D4": 2}
+_DEVICE_TYPES = {"CPE3": 0, "AC2D3": 1, "C3D4": 2, "ASI2D2": 3}
 
 
 def find_dof_in_use_and_max_dof_per_el(unique_typs, dof_to_use=None):
@@ -89,11 +93,24 @@ def fn_build_global_matrices_v5(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, f
     dof_to_use = fe_options["dof_to_use"]
     unique_typs, typ_code = _type_codes(el_typ_i)
     backend = fe_options["assembly_backend"]
+    device_ok = set(unique_typs) <= set(_DEVICE_TYPES)
+    if backend == "auto" and device_ok and els.shape[0] > 200_000:
+        try:
+            return _assemble_device_with_options(nds, els, el_mat_i, el_abs_i, el_typ_i,
+                                                 unique_typs, typ_code, matls, fe_options)
+        except _no_gpu_errors() as e:  # no GPU (or no library): assemble on the host
+            if getattr(e, "status", None) not in (None, _hip_status()):
+                raise
     native_ok = set(unique_typs) <= set(_NATIVE_TYPES) and not fe_options["interface_damping_factor"]
     if backend == "native" and not native_ok:
         raise ValueError("native assembly supports CPE3, AC2D3 and C3D4 elements only")
     if backend == "native" or (backend == "auto" and native_ok and els.shape[0] > 2_000_000):
         return _assemble_native(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls, fe_options)
+    if backend == "device":
+        if not device_ok:
+            raise ValueError("device assembly supports CPE3, AC2D3, C3D4 and ASI2D2 elements only")
+        return _assemble_device_with_options(nds, els, el_mat_i, el_abs_i, el_typ_i, unique_typs,
+                                             typ_code, matls, fe_options)
     unique_df, _ = find_dof_in_use_and_max_dof_per_el(unique_typs, dof_to_use)
     max_df = int(unique_df.max())
     no_nds = nds.shape[0]
@@ -218,6 +235,72 @@ def _assemble_native(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls,
     M = sp.diags(m, format="csr")
     C = sp.diags(c, format="csr")
     C.eliminate_zeros()
+    return K, C, M, gl
+
+
+def _material_tables(matls):
+    D = np.zeros((max(len(matls), 1), 36))
+    rho = np.zeros(max(len(matls), 1))
+    for i, m in enumerate(matls):
+        Dm, r = _material(matls, i + 1)
+        Dm = np.asarray(Dm, dtype=np.float64)
+        D[i, :Dm.size] = Dm.ravel() if Dm.ndim else [float(Dm)]
+        rho[i] = r
+    return D, rho
+
+
+def _els0(els):
+    nn = max(3, els.shape[1])
+    els0 = np.full((els.shape[0], nn), -1, dtype=np.int32)
+    for s0 in range(0, els.shape[0], _CHUNK):
+        e = els[s0:s0 + _CHUNK]
+        els0[s0:s0 + e.shape[0], :els.shape[1]] = np.where(e > 0, e - 1, -1)
+    return els0
+
+
+def _no_gpu_errors():
+    from . import _lib
+    return (_lib.BfeError, OSError, RuntimeError)
+
+
+def _hip_status():
+    from . import _lib
+    return _lib.BFE_ERR_HIP
+
+
+def _assemble_device_with_options(nds, els, el_mat_i, el_abs_i, el_typ_i, unique_typs, typ_code,
+                                  matls, fe_options):
+    K, C, M, gl = _assemble_device(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls,
+                                   fe_options)
+    if fe_options["interface_damping_factor"]:
+        C = fn_add_interface_damping(els, el_typ_i, K, C, M, gl,
+                                     fe_options["interface_damping_factor"])
+    return K, C, M, gl
+
+
+def _assemble_device(nds, els, el_mat_i, el_abs_i, unique_typs, typ_code, matls, fe_options):
+    """GPU assembly (bfe_assemble_device): K and C as CSR, M diagonal."""
+    from . import _lib
+
+    typ = np.asarray([_DEVICE_TYPES[t] for t in unique_typs], dtype=np.int32)[typ_code]
+    solid = typ != _DEVICE_TYPES["ASI2D2"]
+    if np.any(el_mat_i[solid] < 1):
+        raise ValueError("device assembly needs a material for every non-interface element")
+    D, rho = _material_tables(matls)
+    mask = 0
+    for d in (fe_options["dof_to_use"] or []):
+        mask |= 1 << (int(d) - 1)
+    kr, kc, kv, cr, cc, cv, m, gl = _lib.assemble_device(
+        nds, _els0(els), typ, np.where(solid, el_mat_i - 1, -1), el_abs_i, D, rho, mask,
+        fe_options["damping_power_law"], fe_options["max_damping"],
+        fe_options["max_stiffness_reduction"])
+    n = m.size
+    K = sp.csr_matrix((kv, kc, kr), shape=(n, n))
+    K.has_sorted_indices = True
+    C = sp.csr_matrix((cv, cc, cr), shape=(n, n))
+    C.has_sorted_indices = True
+    M = sp.diags(m, format="csr")
+    M.eliminate_zeros()
     return K, C, M, gl
 
 

@@ -401,3 +401,8 @@ bfe_status bfe_assemble_fill(void* work, int64_t* rowptr, int32_t* col, double* 
 const char* bfe_assemble_last_error(void) { return g_asm_error.c_str(); }
 
 }  // extern "C"
+
+namespace bfe {
+// The device assembler (bfe_assemble_dev.hip) reports through the same string.
+void set_assembly_error(const std::string& msg) { g_asm_error = msg; }
+}  // namespace bfe

@@ -346,6 +346,35 @@ bfe_status bfe_assemble_fill(void* work, int64_t* rowptr, int32_t* col, double* 
                              double* c, int64_t* gl_lookup);
 const char* bfe_assemble_last_error(void);
 
+/*
+ * Device-side global assembly (GPU `device`, FP64). Replaces
+ * code/internal/fn_build_global_matrices_v5.m:50-159 with
+ * fn_reduce_global_matrices.m:7-16 and fn_create_fast_lookup.m:4-8 for
+ * meshes of CPE3 (type 0), AC2D3 (1), C3D4 (2) and ASI2D2 fluid-solid
+ * interface elements (3). Inputs as bfe_assemble_pattern; el_mat of an
+ * ASI2D2 element is ignored (the reference gives interface elements no
+ * material, v5:92-95). Node -> element incidence, element matrices and the
+ * row-by-row gather of K and C (each entry summed in ascending element order)
+ * all run on the GPU. bfe_assemble_device returns the sizes and a work object
+ * that holds the result in device memory; bfe_assemble_device_fill copies it
+ * out and releases it: CSR of K (k_rowptr n_dof+1, k_col/k_val nnz_k), CSR of
+ * C (absorbing diagonal and ASI2D2 couplings; c_rowptr, c_col/c_val nnz_c),
+ * the lumped mass m (n_dof) and gl_lookup (n_nodes x max_df, row-major,
+ * 1-based, 0 = absent). bfe_assemble_device_free releases a work object that
+ * is not filled. Errors: bfe_assemble_last_error.
+ */
+bfe_status bfe_assemble_device(int32_t device, int64_t n_nodes, int32_t dim, const double* nds,
+                               int64_t n_el, int32_t nper, const int32_t* els,
+                               const int32_t* el_type, const int32_t* el_mat, const double* el_abs,
+                               int32_t n_mat, const double* mat_D, const double* mat_rho,
+                               int32_t dof_mask, double damping_power_law, double max_damping,
+                               double max_stiffness_reduction, int64_t* n_dof, int64_t* nnz_k,
+                               int64_t* nnz_c, int32_t* max_df, void** work);
+bfe_status bfe_assemble_device_fill(void* work, int64_t* k_rowptr, int32_t* k_col, double* k_val,
+                                    int64_t* c_rowptr, int32_t* c_col, double* c_val, double* m,
+                                    int64_t* gl_lookup);
+bfe_status bfe_assemble_device_free(void* work);
+
 bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
 bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);
 bfe_status bfe_destroy(bfe_handle h);

@@ -1,0 +1,131 @@
+"""Device-side global assembly (bfe_assemble_device) against the host
+assemblers and the oracle's per-element loop
+(code/internal/fn_build_global_matrices_v5.m:50-159)."""
+import numpy as np
+import pytest
+
+from bristolfe_v2_amd import _lib
+from conftest import PKG, assemble, coupled_block, rel_l2, solid_plate, steel
+from oracle import assembly as oa
+from oracle import v6 as ov6
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(A, B, rtol):
+    assert A.shape == B.shape
+    scale = max(abs(A).max(), abs(B).max(), 1e-300)
+    assert abs(A - B).max() <= rtol * scale
+
+
+def _case(case):
+    m = PKG.mesh
+    if case == "solid":
+        mod, matls = solid_plate(2e-3, 1e-3, 0.125e-3)
+        inner = np.array([[0.5e-3, 0.25e-3], [1.5e-3, 0.25e-3], [1.5e-3, 5e-3], [0.5e-3, 5e-3]])
+        mod = m.fn_add_absorbing_layer(mod, inner, 0.25e-3)
+    elif case == "fluid_solid":
+        mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.125e-3)
+    elif case == "fluid_solid_void":
+        void = m.fn_create_smooth_random_blob(0.5, 3, 48, 3) * 0.25e-3 + np.array([1e-3, 0.5e-3])
+        mod, matls = coupled_block(2e-3, 1e-3, 0.75e-3, 0.125e-3, void=void)
+    else:
+        mod = m.cubic_structured_tet_mesh(1e-3, 0.25e-3)
+        mod["el_typ_i"] = np.array(["C3D4"] * len(mod["el_mat_i"]), dtype=object)
+        mod["el_abs_i"] = np.linspace(0, 1, len(mod["el_mat_i"])) ** 2
+        matls = [steel()]
+    return mod, matls
+
+
+@pytest.mark.parametrize("case", ["solid", "fluid_solid", "fluid_solid_void", "tets"])
+@pytest.mark.parametrize("all_dofs", [True, False])
+def test_device_assembly_matches_host_and_oracle(case, all_dofs):
+    print("case: device assembly", case, all_dofs, flush=True)
+    mod, matls = _case(case)
+    dofs = [] if all_dofs else ([1, 2, 4] if case.startswith("fluid") else [1, 2])
+    opts = {"dof_to_use": dofs, "damping_power_law": 2}
+    K1, C1, M1, gl1 = assemble(mod, matls, dict(opts, assembly_backend="numpy"))
+    K2, C2, M2, gl2 = assemble(mod, matls, dict(opts, assembly_backend="device"))
+    np.testing.assert_array_equal(gl1, gl2)
+    for A, B in ((K1, K2), (C1, C2), (M1, M2)):
+        _same(A, B, 1e-13)
+    # the element formulas and summation order of the host native assembler:
+    # equal to the last bit of exp/pow in absorbing elements
+    if "fluid" not in case:
+        K3, C3, M3, gl3 = assemble(mod, matls, dict(opts, assembly_backend="native"))
+        np.testing.assert_array_equal(gl2, gl3)
+        assert K2.nnz == K3.nnz
+        np.testing.assert_array_equal(K2.indptr, K3.indptr)
+        np.testing.assert_array_equal(K2.indices, K3.indices)
+        for A, B in ((K2, K3), (C2, C3), (M2, M3)):
+            _same(A, B, 1e-15)
+    # the oracle's per-element dictionary loop (as in the native test, not on
+    # the masked tets, whose stored pattern depends on rounding-level
+    # cancellations)
+    if case == "tets" and not all_dofs:
+        return
+    Ko, Co, Mo, glo = oa.build_global_matrices(mod["nds"], mod["els"], mod["el_mat_i"],
+                                               mod["el_abs_i"], list(mod["el_typ_i"]), matls, opts)
+    np.testing.assert_array_equal(gl2, glo)
+    for A, B in ((K2, Ko), (C2, Co), (M2, Mo)):
+        _same(A, B, 1e-13)
+
+
+def test_device_assembly_unused_node_and_interface_couplings():
+    print("case: device assembly, unused node", flush=True)
+    mod, matls = coupled_block(1e-3, 0.5e-3, 0.5e-3, 0.125e-3, absorbing=False)
+    mod = dict(mod)
+    mod["nds"] = np.vstack([mod["nds"], [[5e-3, 5e-3]]])  # a node of no element
+    opts = {"dof_to_use": [1, 2, 4]}
+    K, C, M, gl = assemble(mod, matls, dict(opts, assembly_backend="device"))
+    assert np.all(gl[-1] == 0)
+    K1, C1, M1, gl1 = assemble(mod, matls, dict(opts, assembly_backend="numpy"))
+    np.testing.assert_array_equal(gl, gl1)
+    _same(C, C1, 1e-14)
+    # ASI2D2 couplings: C is symmetric, and its off-diagonal entries pair a
+    # solid DOF (u1, u2) with a fluid DOF (p) only
+    assert abs(C - C.T).max() <= 1e-15 * abs(C).max()
+    Cc = C.tocoo()
+    off = Cc.row != Cc.col
+    assert off.any()
+    dof_of = np.zeros(K.shape[0], np.int64)
+    for d in range(gl.shape[1]):
+        g = gl[:, d]
+        dof_of[g[g > 0] - 1] = d + 1
+    pr = dof_of[Cc.row[off]] == 4
+    pc = dof_of[Cc.col[off]] == 4
+    assert np.all(pr ^ pc)
+
+
+def test_device_assembly_rejects_too_many_neighbours():
+    print("case: device assembly, fan of 48 triangles", flush=True)
+    k = 48
+    ang = 2 * np.pi * np.arange(k) / k
+    nds = np.vstack([[0.0, 0.0], np.c_[np.cos(ang), np.sin(ang)] * 1e-3])
+    els = np.array([[1, 2 + i, 2 + (i + 1) % k] for i in range(k)])
+    mod = {"nds": nds, "els": els, "el_mat_i": np.ones(k, np.int64), "el_abs_i": np.zeros(k),
+           "el_typ_i": np.array(["CPE3"] * k, dtype=object)}
+    with pytest.raises(_lib.BfeError, match="neighbouring nodes"):
+        assemble(mod, [steel()], {"dof_to_use": [1, 2], "assembly_backend": "device"})
+
+
+def test_march_on_device_assembled_coupled_model():
+    """The v6 march on matrices assembled on the GPU against the oracle's march
+    on the host-assembled ones (same model: absorbing layer, void, ASI2D2)."""
+    print("case: march on device-assembled matrices", flush=True)
+    m = PKG.mesh
+    void = m.fn_create_smooth_random_blob(0.5, 3, 48, 5) * 0.3e-3 + np.array([1.5e-3, 0.8e-3])
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3, void=void)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+    Kd, Cd, Md, gld = assemble(mod, matls, dict(opts, assembly_backend="device"))
+    Kh, Ch, Mh, glh = assemble(mod, matls, dict(opts, assembly_backend="numpy"))
+    np.testing.assert_array_equal(gld, glh)
+    dt = m.stable_time_step(Kh, Mh.diagonal())
+    t = np.arange(500) * dt
+    F = m.hann_toneburst(t, 5e6, 3)
+    nd = m.fn_find_node_at_point(mod["nds"], [1.5e-3, 2.0e-3])
+    fi = np.array([glh[nd - 1, 3]])
+    hi = np.arange(1, Kh.shape[0] + 1, max(1, Kh.shape[0] // 97))
+    h, _, _, _ = PKG.fn_explicit_dynamic_solver_v6(Kd, Cd, Md, t, fi, F, [], [], hi, np.inf)
+    r, _, _, _ = ov6.explicit_dynamic_solver_v6(Kh, Ch, Mh, t, fi, F, hi)
+    assert rel_l2(h, r) < 1e-10
