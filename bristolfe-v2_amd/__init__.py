@@ -15,3 +15,4 @@ from .driver import fn_BristolFE_v2, fn_FE_entry_point, fn_nds_and_dfs_to_gi  # 
 from .field_output import fn_get_field_output  # noqa: F401
 from .solver import fn_explicit_dynamic_solver_v6, make_plan  # noqa: F401
 from . import mesh  # noqa: F401
+from . import subdoms  # noqa: F401

@@ -36,7 +36,8 @@ EXPORTS = (
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
     "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
-    "bfe_assemble_device_fill", "bfe_assemble_device_free",
+    "bfe_assemble_device_fill", "bfe_assemble_device_free", "bfe_convolve",
+    "bfe_disps_to_forces",
 )
 
 
@@ -155,6 +156,13 @@ def load() -> C.CDLL:
     lib.bfe_assemble_device_fill.restype = C.c_int
     lib.bfe_assemble_device_free.argtypes = [C.c_void_p]
     lib.bfe_assemble_device_free.restype = C.c_int
+    lib.bfe_convolve.argtypes = [C.c_int32, C.c_int64, C.c_int32, _f64p, C.c_int32, _f64p, C.c_int32,
+                                 _f64p]
+    lib.bfe_convolve.restype = C.c_int
+    lib.bfe_disps_to_forces.argtypes = [C.c_int32, C.c_int64, _i64p, _i32p, _f64p, _i64p, _i32p,
+                                        _f64p, _f64p, C.c_double, C.c_int32, _f64p, _i32p,
+                                        C.c_int32, C.c_int32, C.c_int64, _i32p, _f64p]
+    lib.bfe_disps_to_forces.restype = C.c_int
     _lib = lib
     return lib
 
@@ -416,6 +424,48 @@ def assemble_native(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mas
     if st != BFE_OK:
         raise BfeError(st, lib.bfe_assemble_last_error().decode())
     return rowptr, col, kval, m, c, gl
+
+
+def convolve(f, g, g_per_signal, device=0):
+    """bfe_convolve: f (n_sig, n_t) row-major; g (n_g,) or (n_sig, n_g)."""
+    lib = load()
+    f = np.ascontiguousarray(f, np.float64)
+    g = np.ascontiguousarray(g, np.float64)
+    n_sig, n_t = f.shape
+    n_g = g.shape[-1] if g.size else 0
+    h = np.empty((n_sig, n_t), np.float64)
+    check(lib.bfe_convolve(int(device), n_sig, n_t, ptr(f, C.c_double), n_g,
+                           ptr(g, C.c_double) if g.size else None, 1 if g_per_signal else 0,
+                           ptr(h, C.c_double)))
+    return h
+
+
+def disps_to_forces(K, Cm, m, dt, disp, lyrs, in_or_out, solver_mode, device=0):
+    """bfe_disps_to_forces with scipy CSR K, C; disp (n, n_t). Returns
+    (force_idx 0-based, forces (n_force, n_t))."""
+    lib = load()
+    n = K.shape[0]
+    K = K.tocsr()
+    Cm = Cm.tocsr()
+    kp, kc, kv = (np.ascontiguousarray(K.indptr, np.int64), np.ascontiguousarray(K.indices, np.int32),
+                  np.ascontiguousarray(K.data, np.float64))
+    cp, cc, cv = (np.ascontiguousarray(Cm.indptr, np.int64),
+                  np.ascontiguousarray(Cm.indices, np.int32),
+                  np.ascontiguousarray(Cm.data, np.float64))
+    m = np.ascontiguousarray(m, np.float64)
+    disp = np.asfortranarray(disp, np.float64)
+    lyrs = np.ascontiguousarray(lyrs, np.int32).ravel()
+    n_t = disp.shape[1]
+    nf = int(np.count_nonzero((lyrs == 2) | (lyrs == 3)))
+    idx = np.empty(nf, np.int32)
+    out = np.empty((nf, n_t), np.float64, order="F")
+    check(lib.bfe_disps_to_forces(int(device), n, ptr(kp, C.c_int64), ptr(kc, C.c_int32),
+                                  ptr(kv, C.c_double), ptr(cp, C.c_int64), ptr(cc, C.c_int32),
+                                  ptr(cv, C.c_double), ptr(m, C.c_double), float(dt), n_t,
+                                  disp.ctypes.data_as(_f64p), ptr(lyrs, C.c_int32),
+                                  int(in_or_out), int(solver_mode), nf, ptr(idx, C.c_int32),
+                                  out.ctypes.data_as(_f64p)))
+    return idx, out
 
 
 def assemble_device(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mask, p_law,

@@ -61,7 +61,7 @@ def build(verbose: bool = False) -> Path:
         raise RuntimeError(f"hipcc failed:\n{msg}")
     tmp = LIB.with_suffix(".so.tmp")
     link = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp),
-            "-lpthread"]
+            "-lhipfft", "-lpthread"]
     if verbose:
         print(" ".join(link), flush=True)
     r = subprocess.run(link, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)

@@ -986,6 +986,11 @@ struct EbeOp {
 
 }  // namespace
 
+namespace bfe {
+// The subdomain operations (bfe_subdom.hip) report through bfe_last_error.
+void set_subdomain_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace bfe
+
 struct bfe_plan_s {
   int device = 0;
   hipStream_t stream = nullptr;

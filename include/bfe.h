@@ -375,6 +375,35 @@ bfe_status bfe_assemble_device_fill(void* work, int64_t* k_rowptr, int32_t* k_co
                                     int64_t* gl_lookup);
 bfe_status bfe_assemble_device_free(void* work);
 
+/*
+ * Subdomain coupling operations (GPU `device`, FP64).
+ *
+ * bfe_convolve replaces code/internal/fn_convolve.m:22-34: every signal of f
+ * (n_sig x n_t, row-major, time contiguous) is convolved with g through FFTs
+ * of length 2 n_t (g zero-padded or truncated to that length) and the first
+ * n_t samples are kept in h (n_sig x n_t). g is one signal of n_g samples
+ * (g_per_signal = 0) or one per signal (n_sig x n_g, g_per_signal = 1). The
+ * reference's ifft keeps a rounding-level imaginary part; h is the real part.
+ *
+ * bfe_disps_to_forces replaces code/fn_convert_disps_to_forces_v2.m:1-29:
+ * the forces M a + C v + K d of a displacement history disp (n x n_t,
+ * column-major, as MATLAB holds it) on the DOFs of layers 2 and 3 (lyrs, n
+ * values), with the displacements of layers 3, 4 (in_or_out = 0, 'in') or
+ * 1, 2 (1, 'out') set to zero first; m is the lumped mass (sum(M)) and
+ * solver_mode a BFE_MODE_* value choosing the velocity difference. n_force
+ * must equal the number of layer-2 and layer-3 DOFs; force_idx receives them
+ * (0-based, ascending) and force their forces (n_force x n_t, column-major).
+ */
+bfe_status bfe_convolve(int32_t device, int64_t n_sig, int32_t n_t, const double* f, int32_t n_g,
+                        const double* g, int32_t g_per_signal, double* h);
+bfe_status bfe_disps_to_forces(int32_t device, int64_t n, const int64_t* k_rowptr,
+                               const int32_t* k_col, const double* k_val,
+                               const int64_t* c_rowptr, const int32_t* c_col,
+                               const double* c_val, const double* m, double dt, int32_t n_t,
+                               const double* disp, const int32_t* lyrs, int32_t in_or_out,
+                               int32_t solver_mode, int64_t n_force, int32_t* force_idx,
+                               double* force);
+
 bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
 bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);
 bfe_status bfe_destroy(bfe_handle h);
