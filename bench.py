@@ -242,6 +242,7 @@ def bench_fmc(args):
         sel = (fs == s) & (fd < n0)
         r, _, _, _ = ov6.explicit_dynamic_solver_v6(K0, C0, M0, c["time"], fd[sel] + 1, F, hd[q0] + 1)
         orc.append(float(np.linalg.norm(h64[q0, s, :] - r) / np.linalg.norm(r)))
+    sub = subdomain_ops(c, h64, K, C, M)
     info = plan.info()
     speedup = plain_wall / res["double"]["wall"]
     bpd = (12.0 * info["nnz_k"] / n + 4.0) / S + 24.0 + (16.0 / S if info["damped"] else 0.0)
@@ -269,10 +270,62 @@ def bench_fmc(args):
                  "speedup_vs_fp64_batched": round(res["double"]["wall"] / res["single"]["wall"], 3),
                  "autotune": res["single"]["best"]},
         "autotune": {"chosen": res["double"]["best"], "candidates": res["double"]["table"]},
+        "subdomain_ops": sub,
     }
     for r in res.values():
         r["plan"].close()
     print(json.dumps(out), flush=True)
+
+
+def subdomain_ops(c, h64, K, C, M):
+    """The operations around the batched march, timed wall-clock with their
+    host transfers (bfe_convolve, bfe_disps_to_forces) next to the oracle's
+    numpy/scipy versions on the same data: the 64 x 64 pristine FMC A-scans
+    convolved with the input signal, and the boundary forces of a four-layer
+    band of the plate over the whole time axis."""
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    from oracle import subdoms as osd
+    S, n_t = h64.shape[1], h64.shape[2]
+    sig = c["frcs"][0]
+    ascans = np.ascontiguousarray(h64[:S].transpose(1, 0, 2).reshape(S * S, n_t))  # (tx, rx) rows
+    pkg.subdoms.fn_convolve(ascans[:2], sig, 2)   # plan and code-object warm-up
+    w0 = time.perf_counter()
+    fmc = pkg.subdoms.fn_convolve(ascans, sig, 2)
+    conv_s = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    ref = osd.convolve(ascans, sig[None, :], 2)
+    conv_cpu_s = time.perf_counter() - w0
+    conv_err = float(np.abs(fmc - ref).max() / np.abs(ref).max())
+    # boundary band: the plate's DOFs on four node columns (the mesher numbers
+    # nodes up each column, 101 to a column of the 10 mm plate at 0.1 mm),
+    # layers 1..4
+    n0 = c["model_sizes"][0]
+    per_col = 2 * 101
+    band = np.arange(40 * per_col, 44 * per_col)
+    lyrs = 1 + (band - band[0]) // per_col
+    Kb, Cb, Mb = K[band][:, band], C[band][:, band], M[band][:, band]
+    assert band.max() < n0
+    rng = np.random.default_rng(3)
+    disp = rng.standard_normal((band.size, n_t)) * 1e-12
+    pkg.subdoms.fn_convert_disps_to_forces_v2(Kb, Cb, Mb, c["dt"], disp[:, :4], lyrs, "in",
+                                              "vel at curent time step")
+    w0 = time.perf_counter()
+    f, fs, _, _ = pkg.subdoms.fn_convert_disps_to_forces_v2(Kb, Cb, Mb, c["dt"], disp, lyrs, "in",
+                                                            "vel at curent time step")
+    frc_s = time.perf_counter() - w0
+    w0 = time.perf_counter()
+    fo, _, _, _ = osd.disps_to_forces(Kb, Cb, Mb, c["dt"], disp, lyrs, "in",
+                                      "vel at curent time step")
+    frc_cpu_s = time.perf_counter() - w0
+    frc_err = float(np.abs(f - fo).max() / np.abs(fo).max())
+    log(f"subdomain ops: convolve {S * S} x {n_t} in {conv_s:.3f} s (numpy {conv_cpu_s:.3f} s), "
+        f"forces {f.shape} in {frc_s:.3f} s (scipy {frc_cpu_s:.3f} s)")
+    return {"fmc_convolve": {"signals": S * S, "samples": n_t, "wall_s": round(conv_s, 4),
+                             "numpy_fft_wall_s": round(conv_cpu_s, 4), "max_rel_err": conv_err},
+            "disps_to_forces": {"dofs": int(band.size), "force_rows": int(fs.sum()),
+                                "samples": n_t, "wall_s": round(frc_s, 4),
+                                "scipy_wall_s": round(frc_cpu_s, 4), "max_rel_err": frc_err}}
 
 
 def main():
