@@ -434,6 +434,18 @@ __device__ __forceinline__ void ebe_load_records(const int32_t* e0, const int4* 
   }
 }
 
+// a if bit == 1, b if bit == 0, by bit masking (no lane-mask predicate).
+__device__ __forceinline__ double pick_bits(double a, double b, uint32_t bit) {
+  const uint64_t m = 0ull - static_cast<uint64_t>(bit);
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<uint64_t>(__double_as_longlong(a)) & m) |
+      (static_cast<uint64_t>(__double_as_longlong(b)) & ~m)));
+}
+__device__ __forceinline__ float pick_bits(float a, float b, uint32_t bit) {
+  const uint32_t m = 0u - bit;
+  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
 // The update of node v's DPN rows (global rows r0 .. r0 + DPN - 1) from
 // acc = (K u1) on those rows, with the forcing, interface, field and monitor
 // handling of k_step; the lists hold the group's local rows v*DPN + c. alpha
@@ -445,8 +457,11 @@ __device__ __forceinline__ void ebe_finish(
     T* __restrict__ u, const BlockLists& L, const T* __restrict__ f_alpha,
     const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q, T* __restrict__ hist_prev,
     T* __restrict__ fld_prev, int32_t fld_is_vel, T dt, const IfaceRows<T>& I,
-    int32_t coef_per_node) {
+    int32_t coef_per_node, uint32_t undamped = 0u) {
   const int64_t l0 = v * DPN;  // the node's rows in the group's local row numbering
+  // undamped nodes (every DOF with beta = 2 exactly, the rows outside the
+  // absorbing layers) read beta entry 0 instead of their own: one cached
+  // line per wave instead of the beta stream
   T a[DPN], b[DPN], un[DPN];
 #pragma unroll
   for (int c = 0; c < DPN; ++c) {
@@ -456,7 +471,7 @@ __device__ __forceinline__ void ebe_finish(
     b[c] = u2[r];
     const T ka = alpha[ci] * acc[c];
     if constexpr (DAMPED) {
-      const T be = beta[ci];
+      const T be = pick_bits(T(2), beta[ci * (1 - undamped)], undamped);
       un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
     } else {
       un[c] = (T(2) * a[c] - b[c]) - ka;
@@ -572,18 +587,6 @@ struct PatTriIso {  // near-equilateral triangles
                                           {2, 3, 6}, {2, 4, 5}, {2, 5, 6}};
 };
 
-// a if bit == 1, b if bit == 0, by bit masking (no lane-mask predicate).
-__device__ __forceinline__ double pick_bits(double a, double b, uint32_t bit) {
-  const uint64_t m = 0ull - static_cast<uint64_t>(bit);
-  return __longlong_as_double(static_cast<long long>(
-      (static_cast<uint64_t>(__double_as_longlong(a)) & m) |
-      (static_cast<uint64_t>(__double_as_longlong(b)) & ~m)));
-}
-__device__ __forceinline__ float pick_bits(float a, float b, uint32_t bit) {
-  const uint32_t m = 0u - bit;
-  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
-}
-
 // Class offsets (w - v) of a pattern's neighbours on regular nodes.
 template <int NC>
 struct ClassOffsets {
@@ -667,7 +670,8 @@ __device__ __forceinline__ void ebe_pat_node(
       }
     }
     ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
-                               q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+                               q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node,
+                               (pm >> 30) & 1u);
     return;
   }
   T x[NC][DPN];
@@ -716,7 +720,8 @@ __device__ __forceinline__ void ebe_pat_node(
     for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
   }
   ebe_finish<T, DAMPED, DPN>(v, lb, r0, acc, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t,
-                             q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node);
+                             q, hist_prev, fld_prev, fld_is_vel, dt, I, coef_per_node,
+                             (pm >> 30) & 1u);
 }
 
 template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR>
@@ -2155,7 +2160,12 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
           nbrdof[static_cast<size_t>(j) * nn + v] = w < 0 ? -1 : dof0[w];
           if (w >= 0 && (dof0[w] != dof0[v] + Gr.doff[j] || w != v + noff[j])) irregular = true;
         }
-        pm[v] = (pm[v] & 0x7fffffffu) | (irregular ? 0x80000000u : 0u);
+        pm[v] = (pm[v] & 0x3fffffffu) | (irregular ? 0x80000000u : 0u);
+        // bit 30: every DOF of the node undamped (beta = 2 exactly)
+        bool undamped = true;
+        if (h->damped)
+          for (int c = 0; c < dpn; ++c) undamped = undamped && h->beta_host[dof0[v] + c] == 2.0;
+        if (undamped) pm[v] |= 0x40000000u;
       }
       // slots whose element scale is not exactly 1 (absorbing layer)
       std::vector<uint32_t> sm(nn, 0u);
@@ -2171,7 +2181,7 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
       // J_rep / J_e): every occupied slot reads its scale from an anchor table
       int64_t occupied = 0, scaled = 0;
       for (int64_t v = 0; v < nn; ++v) {
-        occupied += __builtin_popcount(pm[v] & 0x7fffffffu);
+        occupied += __builtin_popcount(pm[v] & 0x3fffffffu);
         scaled += __builtin_popcount(sm[v]);
       }
       Gr.smask_all = 2 * scaled > occupied;
