@@ -37,7 +37,7 @@ EXPORTS = (
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
     "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
     "bfe_assemble_device_fill", "bfe_assemble_device_free", "bfe_convolve",
-    "bfe_disps_to_forces",
+    "bfe_disps_to_forces", "bfe_element_templates",
 )
 
 
@@ -163,6 +163,10 @@ def load() -> C.CDLL:
                                         _f64p, _f64p, C.c_double, C.c_int32, _f64p, _i32p,
                                         C.c_int32, C.c_int32, C.c_int64, _i32p, _f64p]
     lib.bfe_disps_to_forces.restype = C.c_int
+    lib.bfe_element_templates.argtypes = [C.c_int32, C.c_int64, C.c_int32, _f64p, C.c_int64,
+                                          C.c_int32, _i32p, _i32p, _i64p, C.c_int32, _i32p, _i64p,
+                                          _i32p, _f64p]
+    lib.bfe_element_templates.restype = C.c_int
     _lib = lib
     return lib
 
@@ -424,6 +428,31 @@ def assemble_native(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mas
     if st != BFE_OK:
         raise BfeError(st, lib.bfe_assemble_last_error().decode())
     return rowptr, col, kval, m, c, gl
+
+
+def element_templates(nds, els0, mat, mult, max_templates, with_jfactor, device=0):
+    """bfe_element_templates: returns (el_tmpl, first, jfactor or None), or
+    None when there are more than max_templates shapes."""
+    lib = load()
+    nds = np.ascontiguousarray(nds, np.float64)
+    els0 = np.ascontiguousarray(els0, np.int32)
+    mat = np.ascontiguousarray(mat, np.int32).ravel()
+    mult = np.ascontiguousarray(mult, np.int64).ravel()
+    n_el = els0.shape[0]
+    el_tmpl = np.empty(n_el, np.int32)
+    first = np.empty(max(int(max_templates), 1), np.int64)
+    jf = np.empty(n_el, np.float64) if with_jfactor else None
+    nt = C.c_int32()
+    st = lib.bfe_element_templates(int(device), nds.shape[0], nds.shape[1], ptr(nds, C.c_double),
+                                   n_el, els0.shape[1], ptr(els0, C.c_int32), ptr(mat, C.c_int32),
+                                   ptr(mult, C.c_int64), int(max_templates),
+                                   ptr(el_tmpl, C.c_int32), ptr(first, C.c_int64), C.byref(nt),
+                                   ptr(jf, C.c_double))
+    if st != BFE_OK:
+        raise BfeError(st, lib.bfe_assemble_last_error().decode())
+    if nt.value == 0:
+        return None
+    return el_tmpl, first[:nt.value].copy(), jf
 
 
 def convolve(f, g, g_per_signal, device=0):

@@ -512,6 +512,122 @@ __global__ void k_rowptr(int64_t n, const int64_t* keep, const int64_t* newidx,
   if (g < n && keep[g]) rowptr[newidx[g]] = pos[g];
 }
 
+// ---------------------------------------------------------------------------
+// Element templates of the element-by-element operator: elements of one type
+// whose node offsets from vertex 1, rounded to 2^-24 of the largest element
+// extent, and material agree share a template. key = sum_i q_i mult_i over
+// the rounded offsets and the material (wrapping 64-bit arithmetic); distinct
+// keys go into a small open-addressing set, each slot keeping its smallest
+// element index (the representative).
+
+constexpr int kTmplSetSize = 4096;  // slots of the key set (power of two)
+constexpr uint64_t kTmplEmpty = ~0ull;
+
+struct TmplMesh {
+  int64_t n_el = 0;
+  int32_t npe = 0, dim = 0;
+  const double* nds = nullptr;
+  const int32_t* els = nullptr;  // n_el x npe, 0-based
+  const int32_t* mat = nullptr;
+};
+
+__global__ void k_tmpl_extent(TmplMesh m, unsigned long long* ext) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double x = 0.0;
+  if (e < m.n_el) {
+    const int32_t* v = m.els + e * m.npe;
+    for (int b = 1; b < m.npe; ++b)
+      for (int c = 0; c < m.dim; ++c)
+        x = fmax(x, fabs(m.nds[static_cast<int64_t>(v[b]) * m.dim + c] -
+                         m.nds[static_cast<int64_t>(v[0]) * m.dim + c]));
+  }
+  // non-negative doubles order as their bit patterns
+  for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_down(x, off));
+  if ((threadIdx.x & 63) == 0)
+    atomicMax(ext, static_cast<unsigned long long>(__double_as_longlong(x)));
+}
+
+__device__ __forceinline__ uint64_t tmpl_key(const TmplMesh& m, int64_t e, double scale,
+                                             const uint64_t* mult) {
+  const int32_t* v = m.els + e * m.npe;
+  uint64_t key = 0;
+  int i = 0;
+  for (int b = 1; b < m.npe; ++b)
+    for (int c = 0; c < m.dim; ++c, ++i) {
+      const double d = m.nds[static_cast<int64_t>(v[b]) * m.dim + c] -
+                       m.nds[static_cast<int64_t>(v[0]) * m.dim + c];
+      key += static_cast<uint64_t>(static_cast<int64_t>(rint(d * scale))) * mult[i];
+    }
+  key += static_cast<uint64_t>(static_cast<int64_t>(m.mat[e])) * mult[i];
+  return key;
+}
+
+// Inserts every element's key; err bit 0: more than max_keys distinct keys,
+// bit 1: a key equal to the empty marker.
+__global__ void k_tmpl_insert(TmplMesh m, double scale, const uint64_t* mult,
+                              unsigned long long* set_key, unsigned long long* set_first,
+                              int32_t* el_slot, int32_t* n_keys, int32_t max_keys, int32_t* err) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m.n_el) return;
+  if (*reinterpret_cast<volatile int32_t*>(err) & 1) return;  // too many already: give up
+  const uint64_t key = tmpl_key(m, e, scale, mult);
+  if (key == kTmplEmpty) {
+    atomicOr(err, 2);
+    return;
+  }
+  uint32_t h = static_cast<uint32_t>((key ^ (key >> 29)) * 0x9E3779B97F4A7C15ull >> 52) &
+               (kTmplSetSize - 1);
+  for (int probe = 0; probe < kTmplSetSize; ++probe) {
+    const unsigned long long prev = atomicCAS(set_key + h, kTmplEmpty, key);
+    if (prev == kTmplEmpty) {  // a new key
+      if (atomicAdd(n_keys, 1) >= max_keys) {
+        atomicOr(err, 1);
+        return;
+      }
+    }
+    if (prev == kTmplEmpty || prev == key) {
+      atomicMin(set_first + h, static_cast<unsigned long long>(e));
+      el_slot[e] = static_cast<int32_t>(h);
+      return;
+    }
+    h = (h + 1) & (kTmplSetSize - 1);
+  }
+  atomicOr(err, 1);
+}
+
+// Template index of every element and, for triangles, J_rep / J_e with the
+// reference's absolute-coordinate J (fn_el_CPE3.m:56); err bit 2: an element
+// whose rounded offsets differ from its representative's (a hash collision).
+#pragma clang fp contract(off)
+__global__ void k_tmpl_assign(TmplMesh m, double scale, const int32_t* slot_tmpl,
+                              const int64_t* first, const int32_t* el_slot, int32_t* el_tmpl,
+                              double* jfac, int32_t* err) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= m.n_el) return;
+  const int32_t t = slot_tmpl[el_slot[e]];
+  el_tmpl[e] = t;
+  const int64_t r = first[t];
+  const int32_t* v = m.els + e * m.npe;
+  const int32_t* w = m.els + r * m.npe;
+  auto X = [&](const int32_t* p, int b, int c) {
+    return m.nds[static_cast<int64_t>(p[b]) * m.dim + c];
+  };
+  bool same = m.mat[e] == m.mat[r];
+  for (int b = 1; b < m.npe; ++b)
+    for (int c = 0; c < m.dim; ++c)
+      same = same && rint((X(v, b, c) - X(v, 0, c)) * scale) == rint((X(w, b, c) - X(w, 0, c)) * scale);
+  if (!same) atomicOr(err, 4);
+  if (jfac != nullptr) {
+    auto J = [&](const int32_t* p) {
+      const double x1 = X(p, 0, 0), y1 = X(p, 0, 1), x2 = X(p, 1, 0), y2 = X(p, 1, 1),
+                   x3 = X(p, 2, 0), y3 = X(p, 2, 1);
+      return x1 * y2 - y1 * x2 - x1 * y3 + y1 * x3 + x2 * y3 - y2 * x3;
+    };
+    jfac[e] = m.npe == 3 ? J(w) / J(v) : 1.0;
+  }
+}
+#pragma clang fp contract(on)
+
 struct DevAssembly {
   int32_t device = 0;
   int64_t n_nodes = 0, n_dof = 0, nnz_k = 0, nnz_c = 0;
@@ -748,6 +864,113 @@ bfe_status bfe_assemble_device_fill(void* work, int64_t* k_rowptr, int32_t* k_co
 bfe_status bfe_assemble_device_free(void* work) {
   delete static_cast<DevAssembly*>(work);
   return BFE_OK;
+}
+
+bfe_status bfe_element_templates(int32_t device, int64_t n_nodes, int32_t dim, const double* nds,
+                                 int64_t n_el, int32_t npe, const int32_t* els,
+                                 const int32_t* el_mat, const int64_t* mult, int32_t max_templates,
+                                 int32_t* el_tmpl, int64_t* first, int32_t* n_tmpl,
+                                 double* jfactor) {
+  try {
+    if (!n_tmpl) throw std::invalid_argument("NULL output");
+    *n_tmpl = 0;
+    if (n_nodes <= 0 || n_el <= 0 || n_el > INT32_MAX || (dim != 2 && dim != 3) || npe < 2 ||
+        npe > 8 || max_templates < 1 || max_templates > kTmplSetSize / 4)
+      throw std::invalid_argument("bad sizes");
+    if (!nds || !els || !el_mat || !mult || !el_tmpl || !first)
+      throw std::invalid_argument("NULL array");
+    if (jfactor != nullptr && (npe != 3 || dim != 2))
+      throw std::invalid_argument("the Jacobian ratio is defined for 2D triangles");
+    for (int64_t i = 0; i < n_el * npe; ++i)
+      if (els[i] < 0 || els[i] >= n_nodes) throw std::invalid_argument("node index out of range");
+    ASM_HIP(hipSetDevice(device));
+    const int nm = (npe - 1) * dim + 1;
+    Buf d_nds = upload(nds, static_cast<size_t>(n_nodes) * dim);
+    Buf d_els = upload(els, static_cast<size_t>(n_el) * npe);
+    Buf d_mat = upload(el_mat, static_cast<size_t>(n_el));
+    Buf d_mult = upload(mult, static_cast<size_t>(nm));
+    TmplMesh m;
+    m.n_el = n_el;
+    m.npe = npe;
+    m.dim = dim;
+    m.nds = d_nds.as<double>();
+    m.els = d_els.as<int32_t>();
+    m.mat = d_mat.as<int32_t>();
+    Buf ext(sizeof(unsigned long long));
+    ASM_HIP(hipMemset(ext.p, 0, ext.bytes));
+    hipLaunchKernelGGL(k_tmpl_extent, dim3(grid(n_el, 256)), dim3(256), 0, nullptr, m,
+                       ext.as<unsigned long long>());
+    ASM_HIP(hipGetLastError());
+    unsigned long long eb = 0;
+    ASM_HIP(hipMemcpy(&eb, ext.p, sizeof(eb), hipMemcpyDeviceToHost));
+    double e_max;
+    std::memcpy(&e_max, &eb, sizeof(e_max));
+    if (!(e_max > 0.0)) throw std::invalid_argument("elements of zero extent");
+    const double scale = 16777216.0 / e_max;  // 2^24 / extent
+    Buf set_key(sizeof(unsigned long long) * kTmplSetSize);
+    Buf set_first(sizeof(unsigned long long) * kTmplSetSize);
+    ASM_HIP(hipMemset(set_key.p, 0xff, set_key.bytes));
+    ASM_HIP(hipMemset(set_first.p, 0xff, set_first.bytes));
+    Buf el_slot(sizeof(int32_t) * n_el), cnt(sizeof(int32_t)), err(sizeof(int32_t));
+    ASM_HIP(hipMemset(cnt.p, 0, cnt.bytes));
+    ASM_HIP(hipMemset(err.p, 0, err.bytes));
+    hipLaunchKernelGGL(k_tmpl_insert, dim3(grid(n_el, 256)), dim3(256), 0, nullptr, m, scale,
+                       d_mult.as<uint64_t>(), set_key.as<unsigned long long>(),
+                       set_first.as<unsigned long long>(), el_slot.as<int32_t>(),
+                       cnt.as<int32_t>(), max_templates, err.as<int32_t>());
+    ASM_HIP(hipGetLastError());
+    int32_t herr = 0;
+    ASM_HIP(hipMemcpy(&herr, err.p, sizeof(herr), hipMemcpyDeviceToHost));
+    if (herr & 1) return BFE_OK;  // more than max_templates shapes: *n_tmpl stays 0
+    if (herr & 2) throw std::invalid_argument("an element key equals the empty marker");
+    std::vector<unsigned long long> hk(kTmplSetSize), hf(kTmplSetSize);
+    ASM_HIP(hipMemcpy(hk.data(), set_key.p, set_key.bytes, hipMemcpyDeviceToHost));
+    ASM_HIP(hipMemcpy(hf.data(), set_first.p, set_first.bytes, hipMemcpyDeviceToHost));
+    // templates in ascending order of the key read as a signed integer
+    std::vector<int32_t> used;
+    for (int32_t h = 0; h < kTmplSetSize; ++h)
+      if (hk[h] != kTmplEmpty) used.push_back(h);
+    std::sort(used.begin(), used.end(), [&](int32_t a, int32_t b) {
+      return static_cast<int64_t>(hk[a]) < static_cast<int64_t>(hk[b]);
+    });
+    std::vector<int32_t> slot_tmpl(kTmplSetSize, -1);
+    std::vector<int64_t> fst(used.size());
+    for (size_t t = 0; t < used.size(); ++t) {
+      slot_tmpl[used[t]] = static_cast<int32_t>(t);
+      fst[t] = static_cast<int64_t>(hf[used[t]]);
+    }
+    Buf d_st = upload(slot_tmpl.data(), slot_tmpl.size());
+    Buf d_first = upload(fst.data(), fst.size());
+    Buf d_tmpl(sizeof(int32_t) * n_el);
+    Buf d_jf;
+    if (jfactor) d_jf = Buf(sizeof(double) * n_el);
+    hipLaunchKernelGGL(k_tmpl_assign, dim3(grid(n_el, 256)), dim3(256), 0, nullptr, m, scale,
+                       d_st.as<int32_t>(), d_first.as<int64_t>(), el_slot.as<int32_t>(),
+                       d_tmpl.as<int32_t>(), jfactor ? d_jf.as<double>() : nullptr,
+                       err.as<int32_t>());
+    ASM_HIP(hipGetLastError());
+    ASM_HIP(hipMemcpy(&herr, err.p, sizeof(herr), hipMemcpyDeviceToHost));
+    if (herr & 4) throw std::invalid_argument("two element shapes share a key (hash collision)");
+    ASM_HIP(hipMemcpy(el_tmpl, d_tmpl.p, sizeof(int32_t) * n_el, hipMemcpyDeviceToHost));
+    if (jfactor)
+      ASM_HIP(hipMemcpy(jfactor, d_jf.p, sizeof(double) * n_el, hipMemcpyDeviceToHost));
+    std::memcpy(first, fst.data(), sizeof(int64_t) * fst.size());
+    *n_tmpl = static_cast<int32_t>(fst.size());
+    return BFE_OK;
+  } catch (const std::invalid_argument& e) {
+    bfe::set_assembly_error(std::string("invalid argument: ") + e.what());
+    return BFE_ERR_ARG;
+  } catch (const dev_failure& e) {
+    (void)hipGetLastError();
+    bfe::set_assembly_error(std::string("HIP error: ") + e.what());
+    return BFE_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    bfe::set_assembly_error("out of memory");
+    return BFE_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    bfe::set_assembly_error(std::string("internal error: ") + e.what());
+    return BFE_ERR_INTERNAL;
+  }
 }
 
 }  // extern "C"

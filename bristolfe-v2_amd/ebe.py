@@ -33,6 +33,7 @@ from .options import set_default_fields
 _NPE = {"CPE3": 3, "AC2D3": 3, "C3D4": 4}
 _CHUNK = 1 << 20
 _MAX_SLOTS = 64
+DEVICE_MIN_ELEMENTS = 200_000  # template detection on the GPU above this many elements
 
 
 @dataclass
@@ -77,9 +78,52 @@ def _unique_type(el_typ_i):
 
 
 def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
-    """(el_tmpl, tmpl) of a set of elements of one type, or None when there
-    are too many distinct shapes. Shapes are node offsets from vertex 1
-    rounded to 2^-24 of the largest element extent, with the material."""
+    """(el_tmpl, tmpl, factor) of a set of elements of one type, or None when
+    there are too many distinct shapes. Shapes are node offsets from vertex 1
+    rounded to 2^-24 of the largest element extent, with the material. Large
+    sets are classified on the GPU (bfe_element_templates) when one is
+    present, with the same keys, order and checks."""
+    n_el = el_nodes.shape[0]
+    rng = np.random.default_rng(12345)
+    mult = rng.integers(1, 2 ** 62, size=(npe - 1) * nds.shape[1] + 1, dtype=np.int64) | 1
+    found = None
+    if n_el > DEVICE_MIN_ELEMENTS:
+        found = _templates_device(nds, el_nodes, mat, mult, npe)
+        if found is False:
+            return None
+    if found is None:
+        found = _templates_host(nds, el_nodes, mat, mult, npe)
+        if found is None:
+            return None
+    inv, first, factor = found
+    from .assembly import _material
+    tmpl = np.zeros((first.size, npe * dpn, npe * dpn))
+    for t, f in enumerate(first):
+        D, rho = _material(matls, int(mat[f]))
+        tmpl[t] = fn(nds, el_nodes[f:f + 1].astype(np.int64), D, rho, dof_to_use)[0][0]
+    return inv.astype(np.int32), tmpl, factor
+
+
+def _templates_device(nds, el_nodes, mat, mult, npe):
+    """(inv, first, factor) from the GPU, False when the set does not qualify
+    (too many shapes, a zero extent, a hash collision), None without a GPU."""
+    from . import _lib
+    try:
+        r = _lib.element_templates(nds, el_nodes, mat, mult, _MAX_SLOTS // npe,
+                                   npe == 3 and nds.shape[1] == 2)
+    except (OSError, RuntimeError) as e:
+        if getattr(e, "status", None) == _lib.BFE_ERR_ARG:
+            return False
+        if getattr(e, "status", None) not in (None, _lib.BFE_ERR_HIP):
+            raise
+        return None
+    if r is None:
+        return False
+    inv, first, jf = r
+    return inv, first, (jf if jf is not None else np.ones(el_nodes.shape[0]))
+
+
+def _templates_host(nds, el_nodes, mat, mult, npe):
     n_el = el_nodes.shape[0]
     ext = 0.0
     for s0 in range(0, n_el, _CHUNK):
@@ -88,8 +132,6 @@ def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
     if not ext > 0:
         return None
     scale = 2.0 ** 24 / ext
-    rng = np.random.default_rng(12345)
-    mult = rng.integers(1, 2 ** 62, size=(npe - 1) * nds.shape[1] + 1, dtype=np.int64) | 1
     keys = np.empty(n_el, dtype=np.int64)
     qs = []
     for s0 in range(0, n_el, _CHUNK):
@@ -106,11 +148,6 @@ def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
     for k, s0 in enumerate(range(0, n_el, _CHUNK)):  # no hash collisions
         if not np.array_equal(qs[k], rep_q[inv[s0:s0 + qs[k].shape[0]]]):
             return None
-    from .assembly import _material
-    tmpl = np.zeros((uk.size, npe * dpn, npe * dpn))
-    for t, f in enumerate(first):
-        D, rho = _material(matls, int(mat[f]))
-        tmpl[t] = fn(nds, el_nodes[f:f + 1].astype(np.int64), D, rho, dof_to_use)[0][0]
     # Triangle matrices are K_e = G(b, c) / J_e with J_e evaluated from absolute
     # coordinates (x1*y2 - y1*x2 - ..., fn_el_CPE3.m:56): far from the origin it
     # carries a cancellation error of about eps*|x|^2/h^2 that the edge vectors
@@ -124,7 +161,7 @@ def _templates(nds, el_nodes, mat, matls, fn, dof_to_use, npe, dpn):
             e = el_nodes[s0:s0 + _CHUNK].astype(np.int64)
             Je = _tri_geometry(nds, e)[0]
             factor[s0:s0 + e.shape[0]] = Jrep[inv[s0:s0 + e.shape[0]]] / Je
-    return inv.astype(np.int32), tmpl, factor
+    return inv, first, factor
 
 
 def element_operator(mod, matls, fe_options, gl_lookup, n_dof):

@@ -376,6 +376,27 @@ bfe_status bfe_assemble_device_fill(void* work, int64_t* k_rowptr, int32_t* k_co
 bfe_status bfe_assemble_device_free(void* work);
 
 /*
+ * Element templates of the element-by-element operator, found on the GPU
+ * (device). The reference assembles K from every element's own matrix
+ * (code/internal/fn_build_global_matrices_v5.m:105-128); on a structured mesh
+ * most elements are translates of a few shapes, and this finds them. Elements
+ * (els: n_el x npe, 0-based; el_mat: material per element) share a template
+ * when their node offsets from vertex 1, rounded to 2^-24 of the largest
+ * offset, and their materials agree: key = sum of the rounded offsets and the
+ * material times mult ((npe-1)*dim + 1 odd multipliers), wrapping 64-bit.
+ * Templates are numbered in ascending order of the key (as a signed integer);
+ * first[t] is the smallest element index of template t, el_tmpl the template
+ * of every element, and jfactor (2D triangles only, else NULL) J_first / J_e
+ * with J from absolute coordinates (fn_el_CPE3.m:56). More than max_templates
+ * shapes: BFE_OK with *n_tmpl = 0. Two shapes with one key: BFE_ERR_ARG.
+ */
+bfe_status bfe_element_templates(int32_t device, int64_t n_nodes, int32_t dim, const double* nds,
+                                 int64_t n_el, int32_t npe, const int32_t* els,
+                                 const int32_t* el_mat, const int64_t* mult, int32_t max_templates,
+                                 int32_t* el_tmpl, int64_t* first, int32_t* n_tmpl,
+                                 double* jfactor);
+
+/*
  * Subdomain coupling operations (GPU `device`, FP64).
  *
  * bfe_convolve replaces code/internal/fn_convolve.m:22-34: every signal of f

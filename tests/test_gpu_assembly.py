@@ -129,3 +129,80 @@ def test_march_on_device_assembled_coupled_model():
     h, _, _, _ = PKG.fn_explicit_dynamic_solver_v6(Kd, Cd, Md, t, fi, F, [], [], hi, np.inf)
     r, _, _, _ = ov6.explicit_dynamic_solver_v6(Kh, Ch, Mh, t, fi, F, hi)
     assert rel_l2(h, r) < 1e-10
+
+
+def _ebe():
+    import importlib
+    return importlib.import_module("bristolfe_v2_amd.ebe")
+
+
+@pytest.mark.parametrize("case", ["tets", "coupled_solid", "coupled_water"])
+def test_device_templates_match_host(case):
+    """bfe_element_templates against the host classification: template of
+    every element, representatives (key order) and J_rep / J_e, bit for bit."""
+    print("case: device templates", case, flush=True)
+    ebe = _ebe()
+    m = PKG.mesh
+    if case == "tets":
+        mod = m.cubic_structured_tet_mesh(2e-3, 0.25e-3)
+        els, mat, npe = mod["els"] - 1, np.ones(len(mod["els"]), np.int64), 4
+        nds = mod["nds"]
+    else:
+        mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3, void=None)
+        typ = "CPE3" if case == "coupled_solid" else "AC2D3"
+        sel = np.asarray(mod["el_typ_i"]) == typ
+        els, mat, npe = np.asarray(mod["els"])[sel, :3] - 1, np.asarray(mod["el_mat_i"])[sel], 3
+        nds = mod["nds"] + np.array([0.3, 0.2])   # far from the origin: J cancellation
+    rng = np.random.default_rng(12345)
+    mult = rng.integers(1, 2 ** 62, size=(npe - 1) * nds.shape[1] + 1, dtype=np.int64) | 1
+    d = ebe._templates_device(nds, els.astype(np.int32), mat, mult, npe)
+    h = ebe._templates_host(nds, els.astype(np.int32), mat, mult, npe)
+    assert d is not None and d is not False and h is not None
+    np.testing.assert_array_equal(d[0], h[0])
+    np.testing.assert_array_equal(d[1], h[1])
+    np.testing.assert_array_equal(d[2], h[2])
+    assert d[1].size >= 2
+
+
+def test_device_templates_reject_irregular_meshes():
+    print("case: device templates, perturbed mesh", flush=True)
+    ebe = _ebe()
+    mod = PKG.mesh.fn_rectangular_structured_mesh([[0, 0], [2e-3, 0], [2e-3, 1e-3], [0, 1e-3]],
+                                                  0.1e-3)
+    nds = mod["nds"] + np.random.default_rng(1).uniform(-1e-5, 1e-5, mod["nds"].shape)
+    els = (mod["els"] - 1).astype(np.int32)
+    mult = np.random.default_rng(12345).integers(1, 2 ** 62, size=5, dtype=np.int64) | 1
+    assert ebe._templates_device(nds, els, np.ones(len(els), np.int64), mult, 3) is False
+    assert ebe._templates_host(nds, els, np.ones(len(els), np.int64), mult, 3) is None
+
+
+def test_element_operator_with_device_templates_marches_like_csr(monkeypatch):
+    """The coupled model's element operator built with GPU template detection
+    (threshold lowered) marches as the CSR operator does."""
+    print("case: element operator from device templates", flush=True)
+    ebe = _ebe()
+    void = PKG.mesh.fn_create_smooth_random_blob(0.5, 3, 48, 7) * 0.3e-3 + np.array([1.5e-3, 0.7e-3])
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3, void=void)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+    K, C, M, gl = assemble(mod, matls, opts)
+    host = ebe.element_operator(mod, matls, opts, gl, K.shape[0])
+    monkeypatch.setattr(ebe, "DEVICE_MIN_ELEMENTS", 0)
+    dev = ebe.element_operator(mod, matls, opts, gl, K.shape[0])
+    for gh, gd in zip(host.groups, dev.groups):
+        np.testing.assert_array_equal(gh.el_tmpl, gd.el_tmpl)
+        np.testing.assert_array_equal(gh.el_scale, gd.el_scale)
+        np.testing.assert_array_equal(gh.tmpl, gd.tmpl)
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    t = np.arange(300) * dt
+    F = PKG.mesh.hann_toneburst(t, 5e6, 3)
+    nd = PKG.mesh.fn_find_node_at_point(mod["nds"], [1.5e-3, 2.0e-3])
+    fi = np.array([gl[nd - 1, 3]]) - 1
+    hi = np.arange(0, K.shape[0], max(1, K.shape[0] // 97))
+    outs = []
+    for op, kind in ((dev, "ebe"), (None, "csr")):
+        plan = PKG.make_plan(K, C, M, dt, element_op=op, operator=kind)
+        try:
+            outs.append(plan.run(t.size, fi, F, hi)[0])
+        finally:
+            plan.close()
+    assert rel_l2(outs[0], outs[1]) < 1e-10
