@@ -37,7 +37,8 @@ EXPORTS = (
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
     "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
     "bfe_assemble_device_fill", "bfe_assemble_device_free", "bfe_convolve",
-    "bfe_disps_to_forces", "bfe_element_templates",
+    "bfe_disps_to_forces", "bfe_element_templates", "bfe_element_centres",
+    "bfe_points_in_polygon", "bfe_dist_points_to_polygon",
 )
 
 
@@ -167,6 +168,14 @@ def load() -> C.CDLL:
                                           C.c_int32, _i32p, _i32p, _i64p, C.c_int32, _i32p, _i64p,
                                           _i32p, _f64p]
     lib.bfe_element_templates.restype = C.c_int
+    lib.bfe_element_centres.argtypes = [C.c_int64, C.c_int32, _f64p, C.c_int64, C.c_int32, _i64p,
+                                        _f64p]
+    lib.bfe_element_centres.restype = C.c_int
+    lib.bfe_points_in_polygon.argtypes = [C.c_int64, _f64p, C.c_int32, _f64p, _f64p,
+                                          _p(C.c_uint8)]
+    lib.bfe_points_in_polygon.restype = C.c_int
+    lib.bfe_dist_points_to_polygon.argtypes = [C.c_int64, _f64p, C.c_int32, _f64p, _f64p]
+    lib.bfe_dist_points_to_polygon.restype = C.c_int
     _lib = lib
     return lib
 
@@ -428,6 +437,41 @@ def assemble_native(nds, els0, el_type, el_mat0, el_abs, mat_D, mat_rho, dof_mas
     if st != BFE_OK:
         raise BfeError(st, lib.bfe_assemble_last_error().decode())
     return rowptr, col, kval, m, c, gl
+
+
+def element_centres(nds, els):
+    """bfe_element_centres: els 1-based (0 = absent), as the model holds it."""
+    lib = load()
+    nds = np.ascontiguousarray(nds, np.float64)
+    els = np.ascontiguousarray(els, np.int64)
+    if els.ndim == 1:
+        els = els[:, None]
+    out = np.empty((els.shape[0], nds.shape[1]), np.float64)
+    check(lib.bfe_element_centres(nds.shape[0], nds.shape[1], ptr(nds, C.c_double), els.shape[0],
+                                  els.shape[1], ptr(els, C.c_int64), ptr(out, C.c_double)))
+    return out
+
+
+def points_in_polygon(pts, vx, vy):
+    lib = load()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 2)
+    vx = np.ascontiguousarray(vx, np.float64).ravel()
+    vy = np.ascontiguousarray(vy, np.float64).ravel()
+    out = np.empty(pts.shape[0], np.uint8)
+    check(lib.bfe_points_in_polygon(pts.shape[0], ptr(pts, C.c_double), vx.size,
+                                    ptr(vx, C.c_double), ptr(vy, C.c_double),
+                                    ptr(out, C.c_uint8)))
+    return out.astype(bool)
+
+
+def dist_points_to_polygon(pts, bdry):
+    lib = load()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 2)
+    bdry = np.ascontiguousarray(bdry, np.float64).reshape(-1, 2)
+    out = np.empty(pts.shape[0], np.float64)
+    check(lib.bfe_dist_points_to_polygon(pts.shape[0], ptr(pts, C.c_double), bdry.shape[0],
+                                         ptr(bdry, C.c_double), ptr(out, C.c_double)))
+    return out
 
 
 def element_templates(nds, els0, mat, mult, max_templates, with_jfactor, device=0):

@@ -132,8 +132,29 @@ def cubic_structured_tet_mesh(size, el_size, origin=(0.0, 0.0, 0.0)):
 # regions and geometry
 
 
+_NATIVE_MIN = 20_000  # points or elements from which the C++ helpers are used
+
+
+def _native():
+    """libbfe_hip's host helpers, or None when the library is not built."""
+    try:
+        from . import _lib
+        return _lib if _lib.load() is not None else None
+    except (OSError, RuntimeError):
+        return None
+
+
 def inpolygon(px, py, vx, vy):
     """Points inside or on the edge of a closed polygon (MATLAB inpolygon)."""
+    px = np.asarray(px, dtype=np.float64)
+    lib = _native() if px.size >= _NATIVE_MIN else None
+    if lib is not None:
+        pts = np.stack([px.ravel(), np.asarray(py, dtype=np.float64).ravel()], axis=1)
+        return lib.points_in_polygon(pts, vx, vy).reshape(px.shape)
+    return _inpolygon_np(px, py, vx, vy)
+
+
+def _inpolygon_np(px, py, vx, vy):
     px = np.asarray(px, dtype=np.float64)
     py = np.asarray(py, dtype=np.float64)
     vx = np.asarray(vx, dtype=np.float64).ravel()
@@ -166,6 +187,14 @@ def inpolygon(px, py, vx, vy):
 
 def fn_calc_element_centres(nds, els):
     els = np.asarray(els)
+    lib = _native() if els.shape[0] >= _NATIVE_MIN else None
+    if lib is not None:
+        return lib.element_centres(nds, els)
+    return _element_centres_np(nds, els)
+
+
+def _element_centres_np(nds, els):
+    els = np.asarray(els)
     nds = np.asarray(nds)
     out = np.empty((els.shape[0], nds.shape[1]))
     for s0 in range(0, els.shape[0], 1 << 20):  # bounded temporaries on large meshes
@@ -179,6 +208,9 @@ def fn_calc_element_centres(nds, els):
 def fn_elements_in_region(mod, region):
     c = fn_calc_element_centres(mod["nds"], mod["els"])
     region = np.asarray(region, dtype=np.float64)
+    lib = _native() if c.shape[0] >= _NATIVE_MIN else None
+    if lib is not None and c.shape[1] == 2:
+        return lib.points_in_polygon(c, region[:, 0], region[:, 1])
     return inpolygon(c[:, 0], c[:, 1], region[:, 0], region[:, 1])
 
 
@@ -202,6 +234,14 @@ def fn_remove_unused_elements(in_use, *arrays):
 def fn_dist_point_to_bdry_2D(pts, bdry):
     """Distance from each point to a polygon, as the reference computes it:
     the nearest vertex and the two edges adjacent to it (fn_dist_point_to_bdry_2D.m:6-22)."""
+    pts = np.asarray(pts, dtype=np.float64)
+    lib = _native() if pts.shape[0] >= _NATIVE_MIN else None
+    if lib is not None:
+        return lib.dist_points_to_polygon(pts, bdry)
+    return _dist_point_to_bdry_2D_np(pts, bdry)
+
+
+def _dist_point_to_bdry_2D_np(pts, bdry):
     pts = np.asarray(pts, dtype=np.float64)
     bdry = np.asarray(bdry, dtype=np.float64)
     nb = bdry.shape[0]

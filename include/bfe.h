@@ -10,8 +10,9 @@
  * (v6:122-123), and dt = time(2) - time(1) (v6:93).
  *
  * Conventions
- *   - Every index crossing this ABI is 0-based. Row pointers are int64,
- *     column and DOF indices int32.
+ *   - Every index crossing this ABI is 0-based (row pointers int64, column
+ *     and DOF indices int32), except in the host mesh-preprocessing helpers,
+ *     which take element tables as the model holds them (1-based int64).
  *   - Dense time data are column-major, as MATLAB stores them:
  *       frcs  : frc_rows x n_t,  element (r, t) at frcs[r + frc_rows * t]
  *       hist  : q x n_t
@@ -424,6 +425,25 @@ bfe_status bfe_disps_to_forces(int32_t device, int64_t n, const int64_t* k_rowpt
                                const double* disp, const int32_t* lyrs, int32_t in_or_out,
                                int32_t solver_mode, int64_t n_force, int32_t* force_idx,
                                double* force);
+
+/*
+ * Host mesh preprocessing (C++, multi-threaded), for models of millions of
+ * elements. Element tables are taken as the model holds them: n_el x nper,
+ * row-major, 1-based node numbers, 0 = absent. Points are n_pts x 2,
+ * row-major.
+ *   bfe_element_centres        code/fn_calc_element_centres.m (out n_el x dim)
+ *   bfe_points_in_polygon      inpolygon as code/fn_elements_in_region.m:6
+ *                              uses it: 1 inside or on the polygon (vx, vy:
+ *                              n_v vertices), else 0
+ *   bfe_dist_points_to_polygon code/fn_dist_point_to_bdry_2D.m: distance to
+ *                              the nearest vertex and to the two edges there
+ */
+bfe_status bfe_element_centres(int64_t n_nodes, int32_t dim, const double* nds, int64_t n_el,
+                               int32_t nper, const int64_t* els, double* out);
+bfe_status bfe_points_in_polygon(int64_t n_pts, const double* pts, int32_t n_v,
+                                 const double* vx, const double* vy, uint8_t* out);
+bfe_status bfe_dist_points_to_polygon(int64_t n_pts, const double* pts, int32_t n_b,
+                                      const double* bdry, double* out);
 
 bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
 bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);
