@@ -563,36 +563,48 @@ __device__ __forceinline__ uint64_t tmpl_key(const TmplMesh& m, int64_t e, doubl
 }
 
 // Inserts every element's key; err bit 0: more than max_keys distinct keys,
-// bit 1: a key equal to the empty marker.
+// bit 1: a key equal to the empty marker. The lanes of a wave share their
+// keys first: one lane per distinct key (the lowest, whose element is the
+// smallest) inserts it, so a wave of one template costs one probe sequence
+// and one atomicMin instead of 64 on the same slot.
 __global__ void k_tmpl_insert(TmplMesh m, double scale, const uint64_t* mult,
                               unsigned long long* set_key, unsigned long long* set_first,
                               int32_t* el_slot, int32_t* n_keys, int32_t max_keys, int32_t* err) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= m.n_el) return;
-  if (*reinterpret_cast<volatile int32_t*>(err) & 1) return;  // too many already: give up
-  const uint64_t key = tmpl_key(m, e, scale, mult);
-  if (key == kTmplEmpty) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  bool pending = e < m.n_el && !(*reinterpret_cast<volatile int32_t*>(err) & 1);
+  const uint64_t key = pending ? tmpl_key(m, e, scale, mult) : 0ull;
+  if (pending && key == kTmplEmpty) {
     atomicOr(err, 2);
-    return;
+    pending = false;
   }
-  uint32_t h = static_cast<uint32_t>((key ^ (key >> 29)) * 0x9E3779B97F4A7C15ull >> 52) &
-               (kTmplSetSize - 1);
-  for (int probe = 0; probe < kTmplSetSize; ++probe) {
-    const unsigned long long prev = atomicCAS(set_key + h, kTmplEmpty, key);
-    if (prev == kTmplEmpty) {  // a new key
-      if (atomicAdd(n_keys, 1) >= max_keys) {
-        atomicOr(err, 1);
-        return;
+  for (;;) {
+    const unsigned long long todo = __ballot(pending);
+    if (todo == 0ull) break;
+    const int leader = __ffsll(static_cast<long long>(todo)) - 1;
+    const uint64_t lk = __shfl(key, leader);
+    int32_t slot = -1;
+    if (lane == leader) {
+      uint32_t h = static_cast<uint32_t>((lk ^ (lk >> 29)) * 0x9E3779B97F4A7C15ull >> 52) &
+                   (kTmplSetSize - 1);
+      for (int probe = 0; probe < kTmplSetSize; ++probe) {
+        const unsigned long long prev = atomicCAS(set_key + h, kTmplEmpty, lk);
+        if (prev == kTmplEmpty && atomicAdd(n_keys, 1) >= max_keys) break;  // too many
+        if (prev == kTmplEmpty || prev == lk) {
+          atomicMin(set_first + h, static_cast<unsigned long long>(e));
+          slot = static_cast<int32_t>(h);
+          break;
+        }
+        h = (h + 1) & (kTmplSetSize - 1);
       }
+      if (slot < 0) atomicOr(err, 1);
     }
-    if (prev == kTmplEmpty || prev == key) {
-      atomicMin(set_first + h, static_cast<unsigned long long>(e));
-      el_slot[e] = static_cast<int32_t>(h);
-      return;
+    slot = __shfl(slot, leader);
+    if (pending && key == lk) {
+      if (slot >= 0) el_slot[e] = slot;
+      pending = false;
     }
-    h = (h + 1) & (kTmplSetSize - 1);
   }
-  atomicOr(err, 1);
 }
 
 // Template index of every element and, for triangles, J_rep / J_e with the
