@@ -1149,9 +1149,11 @@ struct KernelSampler {
 
 template <typename T>
 void upload_plan(bfe_plan_s& H, const bfe::Plan& P) {
-  H.off = upload(P.op.off);
-  H.col = upload(P.op.col);
-  H.val = upload(cast_vec<T>(P.op.val));
+  if (!P.op.off.empty()) {  // SELL built on the host
+    H.off = upload(P.op.off);
+    H.col = upload(P.op.col);
+    H.val = upload(cast_vec<T>(P.op.val));
+  }
   if (P.damped) H.beta = upload(cast_vec<T>(P.beta));
   const int64_t n = P.n;
   const int64_t nb_s = (n + kRowsSingle - 1) / kRowsSingle;
@@ -1171,6 +1173,74 @@ void upload_plan(bfe_plan_s& H, const bfe::Plan& P) {
       H.xval = upload(cast_vec<T>(P.xinv.val));
     }
   }
+}
+
+// SELL-64 of alpha-scaled K built on the device from K's CSR (the layout of
+// bfe_plan.cpp's host builder, value for value): slice widths, then one
+// thread per (slice, lane) writes its row's entries and padding.
+__global__ void k_sell_width(int64_t n, int64_t n_slices, const int64_t* __restrict__ kp,
+                             int64_t* __restrict__ width) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n_slices) return;
+  int64_t w = 0;
+  for (int64_t r = s * 64; r < min(n, s * 64 + 64); ++r) w = max(w, kp[r + 1] - kp[r]);
+  width[s] = w;
+}
+
+template <typename T>
+__global__ void k_sell_fill(int64_t n, int64_t n_slices, const int64_t* __restrict__ kp,
+                            const int32_t* __restrict__ kc, const double* __restrict__ kv,
+                            const double* __restrict__ alpha, const int64_t* __restrict__ off,
+                            int32_t* __restrict__ col, T* __restrict__ val) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = r >> 6;
+  if (s >= n_slices) return;
+  const int lane = static_cast<int>(r & 63);
+  const int64_t w = (off[s + 1] - off[s]) / 64;
+  const int64_t b = r < n ? kp[r] : 0, e = r < n ? kp[r + 1] : 0;
+  const int32_t pad = static_cast<int32_t>(min(r, n - 1));
+  for (int64_t k = 0; k < w; ++k) {
+    const int64_t o = off[s] + 64 * k + lane;
+    if (b + k < e) {
+      col[o] = kc[b + k];
+      val[o] = static_cast<T>(alpha[r] * kv[b + k]);
+    } else {
+      col[o] = pad;
+      val[o] = T(0);
+    }
+  }
+}
+
+template <typename T>
+void build_sell_device(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<double>& alpha) {
+  const int64_t n = K.n, ns = (n + 63) / 64, nnz = K.nnz();
+  DevBuf kp = upload(std::vector<int64_t>(K.ptr, K.ptr + n + 1));
+  DevBuf kc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)), kv(sizeof(double) * std::max<int64_t>(nnz, 1));
+  if (nnz > 0) {
+    BFE_HIP(hipMemcpy(kc.p, K.col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice));
+    BFE_HIP(hipMemcpy(kv.p, K.val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+  }
+  DevBuf width(sizeof(int64_t) * ns);
+  hipLaunchKernelGGL(k_sell_width, dim3((ns + 255) / 256), dim3(256), 0, H.stream, n, ns,
+                     kp.as<int64_t>(), width.as<int64_t>());
+  BFE_HIP(hipGetLastError());
+  std::vector<int64_t> off(ns + 1, 0);
+  BFE_HIP(hipMemcpyAsync(off.data() + 1, width.p, sizeof(int64_t) * ns, hipMemcpyDeviceToHost,
+                         H.stream));
+  BFE_HIP(hipStreamSynchronize(H.stream));
+  for (int64_t q = 0; q < ns; ++q) off[q + 1] = off[q] + 64 * off[q + 1];
+  DevBuf d_alpha = upload(alpha);
+  H.off = upload(off);
+  H.col = DevBuf(sizeof(int32_t) * std::max<int64_t>(off[ns], 1));
+  H.val = DevBuf(sizeof(T) * std::max<int64_t>(off[ns], 1));
+  hipLaunchKernelGGL((k_sell_fill<T>), dim3((ns * 64 + 255) / 256), dim3(256), 0, H.stream, n,
+                     ns, kp.as<int64_t>(), kc.as<int32_t>(), kv.as<double>(),
+                     d_alpha.as<double>(), H.off.as<int64_t>(), H.col.as<int32_t>(),
+                     H.val.as<T>());
+  BFE_HIP(hipGetLastError());
+  BFE_HIP(hipStreamSynchronize(H.stream));
+  H.n_slices = ns;
+  H.nnz_sell = off[ns];
 }
 
 // 0-based first column of F with a non-zero (or NaN) entry; -1 if none.
@@ -2359,9 +2429,16 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
       throw hip_failure("no HIP device available");
     }
     if (device < 0 || device >= ndev) throw std::invalid_argument("device ordinal out of range");
-    bfe::Csr K = bfe::normalise_csr(n, k_rowptr, k_col, k_val, "K");
+    // K in place when it is already normal (sorted, no duplicates or exact
+    // zeros: what the assemblers produce), else a normalised copy
+    bfe::Csr Kc;
+    bfe::CsrRef K{n, k_rowptr, k_col, k_val};
+    if (!bfe::csr_is_normal(n, k_rowptr, k_col, k_val)) {
+      Kc = bfe::normalise_csr(n, k_rowptr, k_col, k_val, "K");
+      K = bfe::ref(Kc);
+    }
     bfe::Csr C = bfe::normalise_csr(n, c_rowptr, c_col, c_val, "C");
-    bfe::Plan P = bfe::build_plan(n, K, m, C, dt, mode);
+    bfe::Plan P = bfe::build_plan(n, K, m, C, dt, mode, /*host_sell=*/false);
 
     auto H = std::make_unique<bfe_plan_s>();
     H->device = device;
@@ -2374,8 +2451,6 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->prec = precision;
     H->dt = dt;
     H->damped = P.damped;
-    H->n_slices = P.op.n_slices;
-    H->nnz_sell = P.op.off.back();
     H->nnz_k = P.nnz_k;
     H->nI = static_cast<int32_t>(P.iface.size());
     H->nnz_coff = P.coff.nnz();
@@ -2387,8 +2462,13 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->iface_host = P.iface;
     H->m_host.assign(m, m + n);
     if (P.damped) H->beta_host = P.beta;
-    if (precision == BFE_PREC_DOUBLE) upload_plan<double>(*H, P);
-    else upload_plan<float>(*H, P);
+    if (precision == BFE_PREC_DOUBLE) {
+      build_sell_device<double>(*H, K, P.alpha);
+      upload_plan<double>(*H, P);
+    } else {
+      build_sell_device<float>(*H, K, P.alpha);
+      upload_plan<float>(*H, P);
+    }
     *out = H.release();
   });
 }

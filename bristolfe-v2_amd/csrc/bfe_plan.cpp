@@ -65,8 +65,13 @@ Csr normalise_csr(int64_t n, const int64_t* ptr, const int32_t* col,
   const int64_t nnz = ptr[n];
   if (nnz > 0 && (col == nullptr || val == nullptr))
     bad(std::string(name) + ": missing column or value array");
-  for (int64_t p = 0; p < nnz; ++p)
-    if (col[p] < 0 || col[p] >= n) bad(std::string(name) + ": column index out of range");
+  std::vector<char> in_range(static_cast<size_t>(n), 1);
+  parallel_for(n, [&](int64_t i) {
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p)
+      if (col[p] < 0 || col[p] >= n) in_range[i] = 0;
+  });
+  for (int64_t i = 0; i < n; ++i)
+    if (!in_range[i]) bad(std::string(name) + ": column index out of range");
   // First pass: count merged entries per row (sorted, duplicates summed,
   // exact zeros dropped, as MATLAB's sparse() stores them).
   std::vector<int64_t> new_len(n, 0);
@@ -280,8 +285,27 @@ Csr truncated_inverse(const Csr& X, double drop_tol) {
   return out;
 }
 
-Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C, double dt,
-                int mode) {
+bool csr_is_normal(int64_t n, const int64_t* ptr, const int32_t* col, const double* val) {
+  if (ptr == nullptr || ptr[0] != 0) return false;
+  for (int64_t i = 0; i < n; ++i)
+    if (ptr[i + 1] < ptr[i]) return false;
+  if (ptr[n] > 0 && (col == nullptr || val == nullptr)) return false;
+  std::vector<char> ok(static_cast<size_t>(n), 1);
+  parallel_for(n, [&](int64_t i) {
+    for (int64_t p = ptr[i]; p < ptr[i + 1]; ++p)
+      if (col[p] < 0 || col[p] >= n || (p > ptr[i] && col[p] <= col[p - 1]) ||
+          !std::isfinite(val[p]) || val[p] == 0.0) {
+        ok[i] = 0;
+        return;
+      }
+  });
+  for (int64_t i = 0; i < n; ++i)
+    if (!ok[i]) return false;
+  return true;
+}
+
+Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C, double dt,
+                int mode, bool host_sell) {
   if (n <= 0) bad("n must be positive");
   if (!(dt > 0.0) || !std::isfinite(dt)) bad("dt must be positive and finite");
   if (mode != 0 && mode != 1) bad("unknown solver mode");
@@ -349,6 +373,7 @@ Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C, double d
   Sell64& S = P.op;
   S.n = n;
   S.n_slices = (n + 63) / 64;
+  if (host_sell) {
   S.off.assign(S.n_slices + 1, 0);
   for (int64_t s = 0; s < S.n_slices; ++s) {
     int64_t w = 0;
@@ -376,6 +401,7 @@ Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C, double d
       }
     }
   });
+  }
 
   // interface correction
   const int64_t nI = static_cast<int64_t>(P.iface.size());

@@ -28,6 +28,16 @@ struct Csr {
   int64_t nnz() const { return ptr.empty() ? 0 : ptr.back(); }
 };
 
+// Read-only view of a CSR held elsewhere (the caller's arrays, or a Csr).
+struct CsrRef {
+  int64_t n = 0;
+  const int64_t* ptr = nullptr;
+  const int32_t* col = nullptr;
+  const double* val = nullptr;
+  int64_t nnz() const { return ptr ? ptr[n] : 0; }
+};
+inline CsrRef ref(const Csr& c) { return {c.n, c.ptr.data(), c.col.data(), c.val.data()}; }
+
 // Sliced ELLPACK with slice height 64 (one wavefront), no row sorting.
 // Entry k of row r lives at off[r/64] + 64*k + r%64. Padding entries point
 // at the row itself with value 0.
@@ -65,10 +75,17 @@ struct Plan {
 Csr normalise_csr(int64_t n, const int64_t* ptr, const int32_t* col,
                   const double* val, const char* name);
 
-// Builds the whole plan. Throws std::invalid_argument (bad input) or
-// std::domain_error (numerical problem, e.g. m_i == 0).
-Plan build_plan(int64_t n, const Csr& K, const double* m, const Csr& C,
-                double dt, int mode);
+// True when the CSR is already normal: row pointers non-decreasing from 0,
+// columns in range and strictly increasing within each row, every value
+// finite and non-zero (checked in parallel). Such a CSR is used in place.
+bool csr_is_normal(int64_t n, const int64_t* ptr, const int32_t* col, const double* val);
+
+// Builds the whole plan; with host_sell = false the SELL-64 operator is left
+// empty (the caller builds it on the device from K and alpha). Throws
+// std::invalid_argument (bad input) or std::domain_error (numerical
+// problem, e.g. m_i == 0).
+Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C,
+                double dt, int mode, bool host_sell = true);
 
 // Exact inverse of the sparse, diagonally dominated interface block, column
 // by column on growing graph neighbourhoods (dense LU with partial pivoting
