@@ -212,7 +212,12 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // share the slice width): their entry loads, shuffles and u1 gathers are
 // interleaved, R independent chains in flight per lane. The forcing,
 // interface and monitor lists are grouped per workgroup (lb).
-template <typename T, bool DAMPED, int R>
+// P (paired): the operator stores rows 2k and 2k+1 over the union of their
+// columns (the two displacement rows of a 2D solid node; explicit zeros
+// where a row lacks a column), so one u1 gather serves both rows and the
+// odd rows' column indices are not loaded. The zero terms only move the
+// two-term grouping of a row's sum: results agree to rounding.
+template <typename T, bool DAMPED, int R, bool P>
 __device__ __forceinline__ void multi_rows(
     int64_t lb, int64_t row0, int64_t n, int32_t S, int lane, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
@@ -225,6 +230,8 @@ __device__ __forceinline__ void multi_rows(
   const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries per row of the slice
   // The rows' entries are loaded once, one per lane, and broadcast with
   // cross-lane shuffles: the gathers of u1 then issue back to back.
+  static_assert(!P || R % 2 == 0, "paired rows need an even row count");
+  constexpr int RS = P ? 2 : 1;  // rows served by one gather
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
@@ -238,29 +245,37 @@ __device__ __forceinline__ void multi_rows(
       myc[r] = 0;
       if (lane < cnt) {
         myv[r] = val[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
-        myc[r] = col[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
+        if (r % RS == 0) myc[r] = col[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
       }
     }
     int k = 0;
     for (; k + 2 <= cnt; k += 2) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = 0; r < R; r += RS) {
         const int c0 = __shfl(myc[r], k), c1 = __shfl(myc[r], k + 1);
-        const T v0 = __shfl(myv[r], k), v1 = __shfl(myv[r], k + 1);
         T x0 = T(0), x1 = T(0);
         if (active) {
           x0 = u1[static_cast<int64_t>(c0) * S + lane];
           x1 = u1[static_cast<int64_t>(c1) * S + lane];
         }
-        acc[r] += v0 * x0 + v1 * x1;
+#pragma unroll
+        for (int j = 0; j < RS; ++j) {
+          const T v0 = __shfl(myv[r + j], k), v1 = __shfl(myv[r + j], k + 1);
+          acc[r + j] += v0 * x0 + v1 * x1;
+        }
       }
     }
     for (; k < cnt; ++k) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
+      for (int r = 0; r < R; r += RS) {
         const int c0 = __shfl(myc[r], k);
-        const T v0 = __shfl(myv[r], k);
-        if (active) acc[r] += v0 * u1[static_cast<int64_t>(c0) * S + lane];
+        T x0 = T(0);
+        if (active) x0 = u1[static_cast<int64_t>(c0) * S + lane];
+#pragma unroll
+        for (int j = 0; j < RS; ++j) {
+          const T v0 = __shfl(myv[r + j], k);
+          if (active) acc[r + j] += v0 * x0;
+        }
       }
     }
   }
@@ -323,7 +338,7 @@ __device__ __forceinline__ void multi_rows(
 }
 
 // Wavefront v of the grid marches rows v*R .. v*R + R - 1.
-template <typename T, bool DAMPED, int R>
+template <typename T, bool DAMPED, int R, bool P>
 __global__ __launch_bounds__(512) void k_step_multi(
     int64_t n, int32_t S, int64_t nblk, int32_t remap, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
@@ -335,7 +350,7 @@ __global__ __launch_bounds__(512) void k_step_multi(
   const int lane = threadIdx.x & 63;
   const int64_t row0 = wave * R;
   if (row0 >= n) return;  // uniform across the wavefront
-  multi_rows<T, DAMPED, R>(lb, row0, n, S, lane, off, col, val, beta, u1, u2, u, L, f_alpha, F,
+  multi_rows<T, DAMPED, R, P>(lb, row0, n, S, lane, off, col, val, beta, u1, u2, u, L, f_alpha, F,
                            F_rows, t, q, hist_prev, I);
 }
 
@@ -1037,6 +1052,7 @@ struct bfe_plan_s {
   double sq = 0.0, sr = 0.0;
   std::vector<double> alpha;
   DevBuf off, col, val, beta;
+  DevBuf pair_off, pair_col, pair_val;  // rows 2k, 2k+1 over their column union
   DevBuf iface, cptr, ccol, cval, gfix, xptr, xcol, xval;
   // interface-row lists grouped per workgroup of the single-source march
   DevBuf is_ptr, is_row, is_k;
@@ -1044,6 +1060,8 @@ struct bfe_plan_s {
   int32_t timing_every = 0;
   int32_t multi_rows = 4;   // wavefronts per workgroup of k_step_multi
   int32_t multi_rpw = 1;    // rows per wavefront of k_step_multi
+  int32_t rows_pair = 0;    // row-paired operator built (pair_off/col/val)
+  int32_t multi_paired = 1; // paired gathers in k_step_multi when rows_pair
   int32_t xcd_remap = 1;    // XCD-aware workgroup order
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
   // adjoint support (defined further down)
@@ -1078,7 +1096,8 @@ struct bfe_plan_s {
   int64_t device_bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr,
-                            &xcol, &xval, &is_ptr, &is_row, &is_k})
+                            &xcol, &xval, &is_ptr, &is_row, &is_k, &pair_off, &pair_col,
+                            &pair_val})
       s += static_cast<int64_t>(b->bytes);
     if (ebe) s += ebe->bytes();
     return s;
@@ -1214,8 +1233,11 @@ __global__ void k_sell_fill(int64_t n, int64_t n_slices, const int64_t* __restri
   }
 }
 
+// SELL-64 of alpha-scaled K, built on the device into (off, col, val);
+// returns the stored entry count including padding.
 template <typename T>
-void build_sell_device(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<double>& alpha) {
+int64_t build_sell_device(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<double>& alpha,
+                          DevBuf& off_out, DevBuf& col_out, DevBuf& val_out) {
   const int64_t n = K.n, ns = (n + 63) / 64, nnz = K.nnz();
   DevBuf kp = upload(std::vector<int64_t>(K.ptr, K.ptr + n + 1));
   DevBuf kc(sizeof(int32_t) * std::max<int64_t>(nnz, 1)), kv(sizeof(double) * std::max<int64_t>(nnz, 1));
@@ -1233,17 +1255,33 @@ void build_sell_device(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<do
   BFE_HIP(hipStreamSynchronize(H.stream));
   for (int64_t q = 0; q < ns; ++q) off[q + 1] = off[q] + 64 * off[q + 1];
   DevBuf d_alpha = upload(alpha);
-  H.off = upload(off);
-  H.col = DevBuf(sizeof(int32_t) * std::max<int64_t>(off[ns], 1));
-  H.val = DevBuf(sizeof(T) * std::max<int64_t>(off[ns], 1));
+  off_out = upload(off);
+  col_out = DevBuf(sizeof(int32_t) * std::max<int64_t>(off[ns], 1));
+  val_out = DevBuf(sizeof(T) * std::max<int64_t>(off[ns], 1));
   hipLaunchKernelGGL((k_sell_fill<T>), dim3((ns * 64 + 255) / 256), dim3(256), 0, H.stream, n,
                      ns, kp.as<int64_t>(), kc.as<int32_t>(), kv.as<double>(),
-                     d_alpha.as<double>(), H.off.as<int64_t>(), H.col.as<int32_t>(),
-                     H.val.as<T>());
+                     d_alpha.as<double>(), off_out.as<int64_t>(), col_out.as<int32_t>(),
+                     val_out.as<T>());
   BFE_HIP(hipGetLastError());
   BFE_HIP(hipStreamSynchronize(H.stream));
-  H.n_slices = ns;
-  H.nnz_sell = off[ns];
+  return off[ns];
+}
+
+// The main operator, and the row-paired one for the multi-source march when
+// pairing saves at least a third of the gathers (2D solids: about 40 %).
+constexpr double kPairMaxRatio = 0.65;  // gathers with pairing / without
+
+template <typename T>
+void build_operators(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<double>& alpha) {
+  H.n_slices = (K.n + 63) / 64;
+  H.nnz_sell = build_sell_device<T>(H, K, alpha, H.off, H.col, H.val);
+  const int64_t nu = bfe::csr_pair_union_nnz(K);
+  // one gather per column of a pair's union (nu counts both rows' copies)
+  H.rows_pair = nu > 0 && 0.5 * static_cast<double>(nu) <= kPairMaxRatio * static_cast<double>(K.nnz());
+  if (H.rows_pair) {
+    const bfe::Csr U = bfe::csr_pair_union(K);
+    build_sell_device<T>(H, bfe::ref(U), alpha, H.pair_off, H.pair_col, H.pair_val);
+  }
 }
 
 // 0-based first column of F with a non-zero (or NaN) entry; -1 if none.
@@ -2018,32 +2056,46 @@ void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
   BFE_HIP(hipMemcpy(g_dens, dgM.p, dgM.bytes, hipMemcpyDeviceToHost));
 }
 
-template <typename T, bool D, int R>
+template <typename T, bool D, int R, bool P>
 void launch_multi_r(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int64_t n, int32_t S,
                     const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
                     const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                     const IfaceRows<T>& IR) {
-  hipLaunchKernelGGL((k_step_multi<T, D, R>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
-                     nblk, H.xcd_remap, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+  const DevBuf &o = P ? H.pair_off : H.off, &c = P ? H.pair_col : H.col,
+               &v = P ? H.pair_val : H.val;
+  hipLaunchKernelGGL((k_step_multi<T, D, R, P>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
+                     nblk, H.xcd_remap, o.as<int64_t>(), c.as<int32_t>(), v.as<T>(),
                      D ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
 }
 
+template <typename T, bool D, bool P>
+void launch_multi_d(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int64_t n,
+                    int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
+                    const T* falpha, const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
+                    const IfaceRows<T>& IR) {
+  switch (rpw) {
+    case 1: launch_multi_r<T, D, 1, false>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 2: launch_multi_r<T, D, 2, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 4: launch_multi_r<T, D, 4, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 8: launch_multi_r<T, D, 8, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    case 16: launch_multi_r<T, D, 16, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+    default: throw std::invalid_argument("rows per wavefront must be 1, 2, 4, 8 or 16");
+  }
+}
+
+// Paired gathers when the plan's rows pair up and pairing is switched on.
 template <typename T>
 void launch_multi(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int64_t n,
                   int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
                   const T* falpha, const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                   const IfaceRows<T>& IR) {
-  const int key = rpw * 2 + (H.damped ? 1 : 0);
-  switch (key) {
-    case 2: launch_multi_r<T, false, 1>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 3: launch_multi_r<T, true, 1>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 4: launch_multi_r<T, false, 2>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 5: launch_multi_r<T, true, 2>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 8: launch_multi_r<T, false, 4>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 9: launch_multi_r<T, true, 4>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 16: launch_multi_r<T, false, 8>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 17: launch_multi_r<T, true, 8>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    default: throw std::invalid_argument("rows per wavefront must be 1, 2, 4 or 8");
+  const bool P = H.rows_pair && H.multi_paired;
+  if (H.damped) {
+    if (P) launch_multi_d<T, true, true>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
+    else launch_multi_d<T, true, false>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
+  } else {
+    if (P) launch_multi_d<T, false, true>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
+    else launch_multi_d<T, false, false>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
   }
 }
 
@@ -2466,10 +2518,10 @@ bfe_status bfe_create(int64_t n, const int64_t* k_rowptr, const int32_t* k_col,
     H->m_host.assign(m, m + n);
     if (P.damped) H->beta_host = P.beta;
     if (precision == BFE_PREC_DOUBLE) {
-      build_sell_device<double>(*H, K, P.alpha);
+      build_operators<double>(*H, K, P.alpha);
       upload_plan<double>(*H, P);
     } else {
-      build_sell_device<float>(*H, K, P.alpha);
+      build_operators<float>(*H, K, P.alpha);
       upload_plan<float>(*H, P);
     }
     *out = H.release();
@@ -2666,9 +2718,12 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         h->multi_rows = value;
         break;
       case BFE_TUNE_MULTI_ROWS_PER_WAVE:
-        if (value != 1 && value != 2 && value != 4 && value != 8)
-          throw std::invalid_argument("rows per wavefront must be 1, 2, 4 or 8");
+        if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+          throw std::invalid_argument("rows per wavefront must be 1, 2, 4, 8 or 16");
         h->multi_rpw = value;
+        break;
+      case BFE_TUNE_MULTI_PAIRED:
+        h->multi_paired = value ? 1 : 0;
         break;
       case BFE_TUNE_XCD_REMAP:
         h->xcd_remap = value ? 1 : 0;
@@ -2733,6 +2788,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
       I.ebe_classes = E.groups[0].n_cls;
       I.ebe_groups = static_cast<int32_t>(E.groups.size());
     }
+    I.rows_pair = h->rows_pair;
     *info = I;
   });
 }

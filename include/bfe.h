@@ -76,6 +76,8 @@ typedef struct bfe_info {
   double ebe_check;      /* set-up check: relative step difference to CSR */
   int32_t ebe_classes;   /* neighbour classes of the mesh pattern         */
   int32_t ebe_groups;    /* element groups of the element operator       */
+  int32_t rows_pair;     /* 1 if a row-paired operator was built          */
+  int32_t reserved;
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
@@ -234,9 +236,16 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  */
 #define BFE_TUNE_MULTI_ROWS_PER_BLOCK 1
 #define BFE_TUNE_XCD_REMAP 2
-/* rows each wavefront of the multi-source march handles together (1, 2, 4
- * or 8; default 1): consecutive rows of one 64-row slice, interleaved */
+/* rows each wavefront of the multi-source march handles together (1, 2, 4,
+ * 8 or 16; default 1): consecutive rows of one 64-row slice, interleaved */
 #define BFE_TUNE_MULTI_ROWS_PER_WAVE 6
+/* 1 (default): when the plan holds a row-paired operator (bfe_info
+ * rows_pair: rows 2k and 2k+1 stored over the union of their columns, built
+ * when the unions hold at most 0.65 times the entries of K, as for the two
+ * rows of a 2D solid node) and a wavefront handles an even number of rows, the
+ * multi-source march gathers each neighbour once for both rows; 0: one
+ * gather per row. Results agree to rounding. */
+#define BFE_TUNE_MULTI_PAIRED 8
 /* 1: bfe_run captures its whole time loop (step kernels, interface solve,
  * last snapshot) into one hipGraph and launches that, instead of launching
  * step by step; for small meshes whose steps are launch-latency bound.

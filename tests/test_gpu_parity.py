@@ -214,6 +214,62 @@ def test_batched_tuning_does_not_change_results():
         plan.close()
 
 
+@pytest.mark.parametrize("absorbing", [False, True])
+def test_batched_paired_rows_bit_identical(absorbing):
+    """2D solid plate (two rows per node whose column lists differ only by
+    exact zeros): the paired gathers of the multi-source kernel agree with one
+    gather per row to rounding, for every even row count per wavefront (and a
+    ragged last slice), and the batched columns match the oracle."""
+    print("case: batched paired rows, absorbing", absorbing, flush=True)
+    from bristolfe_v2_amd import _lib
+    mod, matls = solid_plate(1.6e-3, 0.9e-3, 0.1e-3)
+    if absorbing:
+        inner = np.array([[0.3e-3, 0.3e-3], [1.3e-3, 0.3e-3], [1.3e-3, 5e-3], [0.3e-3, 5e-3]])
+        mod = PKG.mesh.fn_add_absorbing_layer(mod, inner, 0.3e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2], "damping_power_law": 2})
+    n = K.shape[0]
+    assert n % 64 != 0  # the last SELL slice is ragged
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    n_t = 150
+    s = PKG.mesh.hann_toneburst(np.arange(n_t) * dt, 5e6, 3)
+    top = np.nonzero(np.isclose(mod["nds"][:, 1], 0.9e-3))[0]
+    src = gl[top[:24], 1] - 1
+    S = src.size
+    hi = np.concatenate([src, np.arange(0, n, 13)])
+    plan = PKG.make_plan(K, C, M, dt)
+    try:
+        assert plan.info()["rows_pair"] == 1
+        plan.set_tuning(_lib.TUNE_MULTI_PAIRED, 0)
+        ref, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+        plan.set_tuning(_lib.TUNE_MULTI_PAIRED, 1)
+        for rpw in (2, 4, 8, 16):
+            for rows in (1, 4):
+                plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, rpw)
+                plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, rows)
+                h, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+                assert rel_l2(h, ref) < 1e-13
+        for c in (0, S - 1):
+            r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, np.arange(n_t) * dt,
+                                                        np.array([src[c] + 1]), s, hi + 1)
+            assert np.linalg.norm(r[c]) > 0
+            assert rel_l2(h[:, c, :], r) < TOL
+    finally:
+        plan.close()
+
+
+def test_rows_pair_detection():
+    """A plan pairs its rows only when that saves at least a third of the
+    gathers: the coupled solid/water model (nodes of 1, 2 and 3 DOFs, so the
+    pairs straddle nodes) does not."""
+    mod, matls = coupled_block(1.5e-3, 0.75e-3, 0.5e-3, 0.125e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4]})
+    plan = PKG.make_plan(K, C, M, 1e-9)
+    try:
+        assert plan.info()["rows_pair"] == 0
+    finally:
+        plan.close()
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_cube_3d_tets_parity(mode):
     """C3D4 cube with side-face absorbing layers, 3 x 3-node patch source,
