@@ -261,8 +261,7 @@ def bench_fmc(args):
                      "frac": round(bpd * n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": None,
                      "kernel": (f"k_step_multi<double> ({S} sources, {res['double']['best']['rpw']} rows "
-                                "per wavefront" + (", paired rows" if res["double"]["best"].get("paired") else "")
-                                + ")"),
+                                f"per wavefront, {res['double']['best']['group']} rows per gather)"),
                      "bytes_per_dof_step_source": round(bpd, 2)},
         "plain_wall_s": round(plain_wall, 3), "batched_wall_s": round(res["double"]["wall"], 3),
         "plain_loop_ms": round(plain_loop, 1), "batched_loop_ms": round(res["double"]["loop_ms"], 1),

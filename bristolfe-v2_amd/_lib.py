@@ -26,7 +26,7 @@ TUNE_EBE_GROUP = 4
 TUNE_EBE_KERNEL = 5
 TUNE_MULTI_ROWS_PER_WAVE = 6
 TUNE_GRAPH = 7
-TUNE_MULTI_PAIRED = 8
+TUNE_MULTI_GROUP = 8
 
 # Every symbol the header declares.
 EXPORTS = (
@@ -51,7 +51,7 @@ class BfeInfo(C.Structure):
         ("device", C.c_int32), ("device_bytes", C.c_int64), ("operator_kind", C.c_int32),
         ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
         ("ebe_elements", C.c_int64), ("ebe_check", C.c_double), ("ebe_classes", C.c_int32),
-        ("ebe_groups", C.c_int32), ("rows_pair", C.c_int32), ("reserved", C.c_int32),
+        ("ebe_groups", C.c_int32), ("multi_group", C.c_int32), ("reserved", C.c_int32),
     ]
 
 

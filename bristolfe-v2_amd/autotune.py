@@ -6,16 +6,16 @@ Tunables (none changes results):
   rpw     matrix rows each wavefront handles together (interleaved): 1, 2, 4,
           8, 16
   remap   XCD-aware workgroup order on/off
-  paired  one u1 gather for the two rows of a node when the rows pair up
-          (bfe_info rows_pair): on/off
+  group   rows per gather group (one u1 gather serves the group's rows over
+          the union of their columns): automatic, 1, 4, 8
   tile    source columns marched together per pass: 64 or 32 (fewer columns
           per pass re-read the operator more often but keep more lanes busy
           per byte of u)
 Each candidate marches a short probe (a prefix of the real time axis) on the
 real operator; the fastest by HIP-event loop time wins. The search is staged:
-rows x rpw first (XCD order on, pairing on, widest tile), then the XCD
-order, then pairing off, then the tile width, each at the best shape found so
-far.
+rows x rpw first (XCD order on, automatic group, widest tile), then the
+group, then the XCD order, then the tile width, each at the best shape found
+so far.
 """
 from __future__ import annotations
 
@@ -49,9 +49,9 @@ def run_tiled(plan, n_t, n_src, frc_dof, frc_src, frcs, hist_dof, tile):
 
 def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
                  rows=(1, 2, 4, 8), rpws=(1, 2, 4, 8, 16), remaps=(1, 0), tiles=(64, 32),
-                 repeats=2):
-    """Returns (best, table): best = dict(rows, rpw, remap, paired, tile,
-    ms_per_step)."""
+                 groups=(1, 4, 8), repeats=2):
+    """Returns (best, table): best = dict(rows, rpw, remap, group, tile,
+    ms_per_step); group is the one the library used."""
     frcs = np.atleast_2d(frcs)
     # the probe must include forced steps: start it at the first non-zero sample
     nz = np.nonzero(np.abs(frcs).sum(axis=0))[0]
@@ -59,18 +59,16 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
     probe = frcs[:, :t0 + probe_steps]
     table = []
 
-    pairs = bool(plan.info()["rows_pair"])
-
-    def measure(r, w, x, pr, tile):
+    def measure(r, w, x, g, tile):
         plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, r)
         plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, w)
         plan.set_tuning(_lib.TUNE_XCD_REMAP, x)
-        plan.set_tuning(_lib.TUNE_MULTI_PAIRED, pr)
+        plan.set_tuning(_lib.TUNE_MULTI_GROUP, g)
         best = np.inf
         for _ in range(repeats):
             _, ms = run_tiled(plan, probe.shape[1], n_src, frc_dof, frc_src, probe, hist_dof, tile)
             best = min(best, ms)
-        rec = dict(rows=r, rpw=w, remap=x, paired=int(pairs and pr and w > 1), tile=tile,
+        rec = dict(rows=r, rpw=w, remap=x, group=int(plan.info()["multi_group"]), tile=tile,
                    ms_per_step=best / probe_steps)
         table.append(rec)
         return rec
@@ -79,18 +77,22 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
         return rec if rec["ms_per_step"] < best["ms_per_step"] else best
 
     widest = min(tiles[0], n_src)
-    best = min((measure(r, w, remaps[0], 1, widest) for r, w in itertools.product(rows, rpws)),
+    best = min((measure(r, w, remaps[0], 0, widest) for r, w in itertools.product(rows, rpws)),
                key=lambda d: d["ms_per_step"])
+    for g in groups:
+        if g == best["group"]:
+            continue
+        w = max(best["rpw"], g)
+        if w in rpws:
+            best = keep(measure(best["rows"], w, remaps[0], g, widest), best)
     for x in remaps[1:]:
-        best = keep(measure(best["rows"], best["rpw"], x, 1, widest), best)
-    if best["paired"]:
-        best = keep(measure(best["rows"], best["rpw"], best["remap"], 0, widest), best)
+        best = keep(measure(best["rows"], best["rpw"], x, best["group"], widest), best)
     for tile in tiles[1:]:
         if tile >= n_src:
             continue
-        best = keep(measure(best["rows"], best["rpw"], best["remap"], best["paired"], tile), best)
+        best = keep(measure(best["rows"], best["rpw"], best["remap"], best["group"], tile), best)
     plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, best["rows"])
     plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, best["rpw"])
     plan.set_tuning(_lib.TUNE_XCD_REMAP, best["remap"])
-    plan.set_tuning(_lib.TUNE_MULTI_PAIRED, best["paired"] if pairs else 1)
+    plan.set_tuning(_lib.TUNE_MULTI_GROUP, best["group"])
     return best, table

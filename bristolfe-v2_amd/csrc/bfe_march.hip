@@ -212,12 +212,27 @@ __global__ __launch_bounds__(kBlock) void k_step(
 // share the slice width): their entry loads, shuffles and u1 gathers are
 // interleaved, R independent chains in flight per lane. The forcing,
 // interface and monitor lists are grouped per workgroup (lb).
-// P (paired): the operator stores rows 2k and 2k+1 over the union of their
-// columns (the two displacement rows of a 2D solid node; explicit zeros
-// where a row lacks a column), so one u1 gather serves both rows and the
-// odd rows' column indices are not loaded. The zero terms only move the
-// two-term grouping of a row's sum: results agree to rounding.
-template <typename T, bool DAMPED, int R, bool P>
+// G (rows per gather group): with G > 1 the operator stores each aligned
+// group of G rows over the union of their columns (explicit zeros where a
+// row lacks a column; G = 2 pairs the two displacement rows of a 2D solid
+// node, G = 4 two neighbouring nodes), so one u1 gather serves G rows and
+// only the first row's column indices are loaded. The zero terms only move
+// the two-term grouping of a row's sum: results agree to rounding.
+// Entries are broadcast from the lane that loaded them with readlane (VALU,
+// into scalar registers), not through the LDS crossbar.
+__device__ __forceinline__ int lane_bcast(int v, int k) { return __builtin_amdgcn_readlane(v, k); }
+__device__ __forceinline__ float lane_bcast(float v, int k) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), k));
+}
+__device__ __forceinline__ double lane_bcast(double v, int k) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), k);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), k);
+  return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) |
+                                        static_cast<long long>(static_cast<unsigned>(lo)));
+}
+
+template <typename T, bool DAMPED, int R, int G>
 __device__ __forceinline__ void multi_rows(
     int64_t lb, int64_t row0, int64_t n, int32_t S, int lane, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
@@ -230,8 +245,8 @@ __device__ __forceinline__ void multi_rows(
   const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries per row of the slice
   // The rows' entries are loaded once, one per lane, and broadcast with
   // cross-lane shuffles: the gathers of u1 then issue back to back.
-  static_assert(!P || R % 2 == 0, "paired rows need an even row count");
-  constexpr int RS = P ? 2 : 1;  // rows served by one gather
+  static_assert(G >= 1 && R % G == 0, "rows per wavefront must be a multiple of the group");
+  constexpr int RS = G;  // rows served by one gather
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
@@ -252,7 +267,7 @@ __device__ __forceinline__ void multi_rows(
     for (; k + 2 <= cnt; k += 2) {
 #pragma unroll
       for (int r = 0; r < R; r += RS) {
-        const int c0 = __shfl(myc[r], k), c1 = __shfl(myc[r], k + 1);
+        const int c0 = lane_bcast(myc[r], k), c1 = lane_bcast(myc[r], k + 1);
         T x0 = T(0), x1 = T(0);
         if (active) {
           x0 = u1[static_cast<int64_t>(c0) * S + lane];
@@ -260,7 +275,7 @@ __device__ __forceinline__ void multi_rows(
         }
 #pragma unroll
         for (int j = 0; j < RS; ++j) {
-          const T v0 = __shfl(myv[r + j], k), v1 = __shfl(myv[r + j], k + 1);
+          const T v0 = lane_bcast(myv[r + j], k), v1 = lane_bcast(myv[r + j], k + 1);
           acc[r + j] += v0 * x0 + v1 * x1;
         }
       }
@@ -268,12 +283,12 @@ __device__ __forceinline__ void multi_rows(
     for (; k < cnt; ++k) {
 #pragma unroll
       for (int r = 0; r < R; r += RS) {
-        const int c0 = __shfl(myc[r], k);
+        const int c0 = lane_bcast(myc[r], k);
         T x0 = T(0);
         if (active) x0 = u1[static_cast<int64_t>(c0) * S + lane];
 #pragma unroll
         for (int j = 0; j < RS; ++j) {
-          const T v0 = __shfl(myv[r + j], k);
+          const T v0 = lane_bcast(myv[r + j], k);
           if (active) acc[r + j] += v0 * x0;
         }
       }
@@ -338,7 +353,7 @@ __device__ __forceinline__ void multi_rows(
 }
 
 // Wavefront v of the grid marches rows v*R .. v*R + R - 1.
-template <typename T, bool DAMPED, int R, bool P>
+template <typename T, bool DAMPED, int R, int G>
 __global__ __launch_bounds__(512) void k_step_multi(
     int64_t n, int32_t S, int64_t nblk, int32_t remap, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
@@ -346,11 +361,13 @@ __global__ __launch_bounds__(512) void k_step_multi(
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
     T* __restrict__ hist_prev, IfaceRows<T> I) {
   const int64_t lb = remap ? xcd_block(blockIdx.x, nblk) : static_cast<int64_t>(blockIdx.x);
-  const int64_t wave = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // wavefront index as a scalar: row and entry offsets stay in SGPRs
+  const int64_t wave =
+      lb * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
   const int64_t row0 = wave * R;
   if (row0 >= n) return;  // uniform across the wavefront
-  multi_rows<T, DAMPED, R, P>(lb, row0, n, S, lane, off, col, val, beta, u1, u2, u, L, f_alpha, F,
+  multi_rows<T, DAMPED, R, G>(lb, row0, n, S, lane, off, col, val, beta, u1, u2, u, L, f_alpha, F,
                            F_rows, t, q, hist_prev, I);
 }
 
@@ -1037,6 +1054,13 @@ namespace bfe {
 void set_subdomain_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace bfe
 
+// A row-grouped SELL-64 operator of the multi-source march (see group_op).
+struct GroupOp {
+  DevBuf off, col, val;
+  int64_t gathers = 0;  // union entries summed over the groups
+  bool built = false;
+};
+
 struct bfe_plan_s {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -1052,7 +1076,6 @@ struct bfe_plan_s {
   double sq = 0.0, sr = 0.0;
   std::vector<double> alpha;
   DevBuf off, col, val, beta;
-  DevBuf pair_off, pair_col, pair_val;  // rows 2k, 2k+1 over their column union
   DevBuf iface, cptr, ccol, cval, gfix, xptr, xcol, xval;
   // interface-row lists grouped per workgroup of the single-source march
   DevBuf is_ptr, is_row, is_k;
@@ -1060,8 +1083,10 @@ struct bfe_plan_s {
   int32_t timing_every = 0;
   int32_t multi_rows = 4;   // wavefronts per workgroup of k_step_multi
   int32_t multi_rpw = 1;    // rows per wavefront of k_step_multi
-  int32_t rows_pair = 0;    // row-paired operator built (pair_off/col/val)
-  int32_t multi_paired = 1; // paired gathers in k_step_multi when rows_pair
+  int32_t multi_group = 0;  // rows per gather group of k_step_multi (0: automatic)
+  int32_t auto_group = 0;   // automatic choice once made (1 or 2)
+  int32_t last_group = 0;   // group of the last batched run
+  GroupOp grp[3];           // row-grouped operators for G = 2, 4, 8 (built on use)
   int32_t xcd_remap = 1;    // XCD-aware workgroup order
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
   // adjoint support (defined further down)
@@ -1096,9 +1121,9 @@ struct bfe_plan_s {
   int64_t device_bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&off, &col, &val, &beta, &iface, &cptr, &ccol, &cval, &gfix, &xptr,
-                            &xcol, &xval, &is_ptr, &is_row, &is_k, &pair_off, &pair_col,
-                            &pair_val})
+                            &xcol, &xval, &is_ptr, &is_row, &is_k})
       s += static_cast<int64_t>(b->bytes);
+    for (const GroupOp& g : grp) s += static_cast<int64_t>(g.off.bytes + g.col.bytes + g.val.bytes);
     if (ebe) s += ebe->bytes();
     return s;
   }
@@ -1267,21 +1292,155 @@ int64_t build_sell_device(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector
   return off[ns];
 }
 
-// The main operator, and the row-paired one for the multi-source march when
-// pairing saves at least a third of the gathers (2D solids: about 40 %).
-constexpr double kPairMaxRatio = 0.65;  // gathers with pairing / without
-
 template <typename T>
 void build_operators(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<double>& alpha) {
   H.n_slices = (K.n + 63) / 64;
   H.nnz_sell = build_sell_device<T>(H, K, alpha, H.off, H.col, H.val);
-  const int64_t nu = bfe::csr_pair_union_nnz(K);
-  // one gather per column of a pair's union (nu counts both rows' copies)
-  H.rows_pair = nu > 0 && 0.5 * static_cast<double>(nu) <= kPairMaxRatio * static_cast<double>(K.nnz());
-  if (H.rows_pair) {
-    const bfe::Csr U = bfe::csr_pair_union(K);
-    build_sell_device<T>(H, bfe::ref(U), alpha, H.pair_off, H.pair_col, H.pair_val);
+}
+
+// ---------------------------------------------------------------------------
+// Row-grouped operators of the multi-source march, built on the device from
+// the SELL-64 operator: each aligned group of G rows (G divides 64, so a
+// group lies in one slice) stored over the union of its rows' columns, in
+// the SELL-64 layout (entry k of row r at off[s] + 64 k + r % 64), explicit
+// zeros where a row lacks a column, padded per slice with value 0. Entries
+// of value 0 in the source (padding) are skipped.
+template <typename T, int G, typename Emit>
+__device__ __forceinline__ int group_merge(int64_t base, int w, const int32_t* __restrict__ col,
+                                           const T* __restrict__ val, Emit&& emit) {
+  int h[G];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    h[j] = 0;
+    while (h[j] < w && val[base + j + 64 * static_cast<int64_t>(h[j])] == T(0)) ++h[j];
   }
+  int len = 0;
+  for (;;) {
+    int c = INT32_MAX;
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (h[j] < w) c = min(c, col[base + j + 64 * static_cast<int64_t>(h[j])]);
+    if (c == INT32_MAX) break;
+    T v[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      v[j] = T(0);
+      if (h[j] < w && col[base + j + 64 * static_cast<int64_t>(h[j])] == c) {
+        v[j] = val[base + j + 64 * static_cast<int64_t>(h[j])];
+        do ++h[j];
+        while (h[j] < w && val[base + j + 64 * static_cast<int64_t>(h[j])] == T(0));
+      }
+    }
+    emit(len, c, v);
+    ++len;
+  }
+  return len;
+}
+
+template <typename T, int G>
+__global__ void k_group_width(int64_t n_slices, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ col, const T* __restrict__ val,
+                              unsigned long long* __restrict__ width,
+                              unsigned long long* __restrict__ total) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = g / (64 / G);
+  if (s >= n_slices) return;
+  const int r0 = static_cast<int>(g % (64 / G)) * G;  // first row of the group in its slice
+  const int w = static_cast<int>((off[s + 1] - off[s]) >> 6);
+  const int len = group_merge<T, G>(off[s] + r0, w, col, val, [](int, int, const T*) {});
+  atomicMax(width + s, static_cast<unsigned long long>(len));
+  atomicAdd(total, static_cast<unsigned long long>(len));
+}
+
+template <typename T, int G>
+__global__ void k_group_fill(int64_t n, int64_t n_slices, const int64_t* __restrict__ off,
+                             const int32_t* __restrict__ col, const T* __restrict__ val,
+                             const int64_t* __restrict__ goff, int32_t* __restrict__ gcol,
+                             T* __restrict__ gval) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = g / (64 / G);
+  if (s >= n_slices) return;
+  const int r0 = static_cast<int>(g % (64 / G)) * G;
+  const int w = static_cast<int>((off[s + 1] - off[s]) >> 6);
+  const int64_t ob = goff[s] + r0;
+  const int wg = static_cast<int>((goff[s + 1] - goff[s]) >> 6);
+  const int len = group_merge<T, G>(off[s] + r0, w, col, val, [&](int k, int c, const T* v) {
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      gcol[ob + j + 64 * static_cast<int64_t>(k)] = c;
+      gval[ob + j + 64 * static_cast<int64_t>(k)] = v[j];
+    }
+  });
+  for (int k = len; k < wg; ++k)
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      gcol[ob + j + 64 * static_cast<int64_t>(k)] =
+          static_cast<int32_t>(min<int64_t>(s * 64 + r0 + j, n - 1));
+      gval[ob + j + 64 * static_cast<int64_t>(k)] = T(0);
+    }
+}
+
+int group_index(int G) { return G == 2 ? 0 : G == 4 ? 1 : 2; }
+
+template <typename T, int G>
+void build_group_op_g(bfe_plan_s& H, GroupOp& op) {
+  const int64_t ns = H.n_slices, ng = ns * (64 / G);
+  DevBuf width(sizeof(unsigned long long) * ns), total(sizeof(unsigned long long));
+  BFE_HIP(hipMemsetAsync(width.p, 0, width.bytes, H.stream));
+  BFE_HIP(hipMemsetAsync(total.p, 0, total.bytes, H.stream));
+  hipLaunchKernelGGL((k_group_width<T, G>), dim3((ng + 255) / 256), dim3(256), 0, H.stream, ns,
+                     H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+                     width.as<unsigned long long>(), total.as<unsigned long long>());
+  BFE_HIP(hipGetLastError());
+  std::vector<int64_t> off(ns + 1, 0);
+  unsigned long long tot = 0;
+  BFE_HIP(hipMemcpyAsync(off.data() + 1, width.p, width.bytes, hipMemcpyDeviceToHost, H.stream));
+  BFE_HIP(hipMemcpyAsync(&tot, total.p, sizeof(tot), hipMemcpyDeviceToHost, H.stream));
+  BFE_HIP(hipStreamSynchronize(H.stream));
+  for (int64_t q = 0; q < ns; ++q) off[q + 1] = off[q] + 64 * off[q + 1];
+  op.off = upload(off);
+  op.col = DevBuf(sizeof(int32_t) * std::max<int64_t>(off[ns], 1));
+  op.val = DevBuf(sizeof(T) * std::max<int64_t>(off[ns], 1));
+  hipLaunchKernelGGL((k_group_fill<T, G>), dim3((ng + 255) / 256), dim3(256), 0, H.stream, H.n,
+                     ns, H.off.as<int64_t>(), H.col.as<int32_t>(), H.val.as<T>(),
+                     op.off.as<int64_t>(), op.col.as<int32_t>(), op.val.as<T>());
+  BFE_HIP(hipGetLastError());
+  BFE_HIP(hipStreamSynchronize(H.stream));
+  op.gathers = static_cast<int64_t>(tot);
+  op.built = true;
+}
+
+// The grouped operator for G (2, 4 or 8), built on first use.
+template <typename T>
+const GroupOp& group_op(bfe_plan_s& H, int G) {
+  GroupOp& op = H.grp[group_index(G)];
+  if (!op.built) {
+    if (G == 2) build_group_op_g<T, 2>(H, op);
+    else if (G == 4) build_group_op_g<T, 4>(H, op);
+    else build_group_op_g<T, 8>(H, op);
+  }
+  return op;
+}
+
+// Pairing pays when it saves at least a third of the gathers (2D solids:
+// about 40 %); the automatic choice is G = 2 then, else G = 1.
+constexpr double kPairMaxRatio = 0.65;
+
+template <typename T>
+int resolve_group(bfe_plan_s& H, int rpw) {
+  int G = H.multi_group;
+  if (G == 0) {
+    if (H.auto_group == 0) {
+      const GroupOp& op = group_op<T>(H, 2);
+      H.auto_group = static_cast<double>(op.gathers) <= kPairMaxRatio * static_cast<double>(H.nnz_k)
+                         ? 2 : 1;
+      if (H.auto_group == 1) H.grp[group_index(2)] = GroupOp{};
+    }
+    G = H.auto_group;
+  }
+  G = std::min(G, rpw);
+  if (G > 1) group_op<T>(H, G);
+  return G;
 }
 
 // 0-based first column of F with a non-zero (or NaN) entry; -1 if none.
@@ -2056,47 +2215,49 @@ void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
   BFE_HIP(hipMemcpy(g_dens, dgM.p, dgM.bytes, hipMemcpyDeviceToHost));
 }
 
-template <typename T, bool D, int R, bool P>
+template <typename T, bool D, int R, int G>
 void launch_multi_r(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int64_t n, int32_t S,
                     const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
                     const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                     const IfaceRows<T>& IR) {
-  const DevBuf &o = P ? H.pair_off : H.off, &c = P ? H.pair_col : H.col,
-               &v = P ? H.pair_val : H.val;
-  hipLaunchKernelGGL((k_step_multi<T, D, R, P>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
+  const GroupOp* g = G > 1 ? &H.grp[G == 2 ? 0 : G == 4 ? 1 : 2] : nullptr;
+  const DevBuf &o = g ? g->off : H.off, &c = g ? g->col : H.col, &v = g ? g->val : H.val;
+  hipLaunchKernelGGL((k_step_multi<T, D, R, G>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
                      nblk, H.xcd_remap, o.as<int64_t>(), c.as<int32_t>(), v.as<T>(),
                      D ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
 }
 
-template <typename T, bool D, bool P>
-void launch_multi_d(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int64_t n,
-                    int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
+// Rows per wavefront R and rows per gather group G (G <= R, both powers of 2).
+template <typename T, bool D>
+void launch_multi_d(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int32_t G,
+                    int64_t n, int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
                     const T* falpha, const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                     const IfaceRows<T>& IR) {
-  switch (rpw) {
-    case 1: launch_multi_r<T, D, 1, false>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 2: launch_multi_r<T, D, 2, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 4: launch_multi_r<T, D, 4, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 8: launch_multi_r<T, D, 8, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
-    case 16: launch_multi_r<T, D, 16, P>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR); break;
+#define BFE_MULTI(R_, G_)                                                                   \
+  case R_ * 16 + G_:                                                                        \
+    launch_multi_r<T, D, R_, G_>(H, nblk, rpb, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, \
+                                 hp, IR);                                                   \
+    break;
+  switch (rpw * 16 + G) {
+    BFE_MULTI(1, 1)
+    BFE_MULTI(2, 1) BFE_MULTI(2, 2)
+    BFE_MULTI(4, 1) BFE_MULTI(4, 2) BFE_MULTI(4, 4)
+    BFE_MULTI(8, 1) BFE_MULTI(8, 2) BFE_MULTI(8, 4) BFE_MULTI(8, 8)
+    BFE_MULTI(16, 1) BFE_MULTI(16, 2) BFE_MULTI(16, 4) BFE_MULTI(16, 8)
     default: throw std::invalid_argument("rows per wavefront must be 1, 2, 4, 8 or 16");
   }
+#undef BFE_MULTI
 }
 
-// Paired gathers when the plan's rows pair up and pairing is switched on.
 template <typename T>
-void launch_multi(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int64_t n,
-                  int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
+void launch_multi(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, int32_t G,
+                  int64_t n, int32_t S, const T* u1, const T* u2, T* u, const BlockLists& L,
                   const T* falpha, const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                   const IfaceRows<T>& IR) {
-  const bool P = H.rows_pair && H.multi_paired;
-  if (H.damped) {
-    if (P) launch_multi_d<T, true, true>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
-    else launch_multi_d<T, true, false>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
-  } else {
-    if (P) launch_multi_d<T, false, true>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
-    else launch_multi_d<T, false, false>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
-  }
+  if (H.damped)
+    launch_multi_d<T, true>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
+  else
+    launch_multi_d<T, false>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
 }
 
 template <typename T>
@@ -2122,6 +2283,8 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   hipStream_t st = H.stream;
   const int32_t rpb = H.multi_rows;
   const int32_t rpw = H.multi_rpw;
+  const int32_t G = resolve_group<T>(H, rpw);
+  H.last_group = G;
   const int64_t rows_per_wg = static_cast<int64_t>(rpb) * rpw;
   const int64_t nblk = (n + rows_per_wg - 1) / rows_per_wg;
   RunBuffers R;
@@ -2157,7 +2320,7 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     T* hp = (q > 0 && prev >= t0) ? hist + prev * per_t : nullptr;
     const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
-    launch_multi<T>(H, nblk, rpb, rpw, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
+    launch_multi<T>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
     sampler.after(t);
     ++launches;
     if (H.nI > 0 && H.has_xinv) {
@@ -2722,8 +2885,10 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
           throw std::invalid_argument("rows per wavefront must be 1, 2, 4, 8 or 16");
         h->multi_rpw = value;
         break;
-      case BFE_TUNE_MULTI_PAIRED:
-        h->multi_paired = value ? 1 : 0;
+      case BFE_TUNE_MULTI_GROUP:
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8)
+          throw std::invalid_argument("rows per gather group must be 0 (automatic), 1, 2, 4 or 8");
+        h->multi_group = value;
         break;
       case BFE_TUNE_XCD_REMAP:
         h->xcd_remap = value ? 1 : 0;
@@ -2788,7 +2953,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
       I.ebe_classes = E.groups[0].n_cls;
       I.ebe_groups = static_cast<int32_t>(E.groups.size());
     }
-    I.rows_pair = h->rows_pair;
+    I.multi_group = h->last_group;
     *info = I;
   });
 }

@@ -304,63 +304,6 @@ bool csr_is_normal(int64_t n, const int64_t* ptr, const int32_t* col, const doub
   return true;
 }
 
-namespace {
-
-// Size of the merged column list of rows a and b (both sorted ascending).
-int64_t merged_len(const CsrRef& K, int64_t a, int64_t b) {
-  int64_t i = K.ptr[a], ie = K.ptr[a + 1], j = K.ptr[b], je = K.ptr[b + 1], m = 0;
-  while (i < ie && j < je) {
-    const int32_t ci = K.col[i], cj = K.col[j];
-    i += ci <= cj;
-    j += cj <= ci;
-    ++m;
-  }
-  return m + (ie - i) + (je - j);
-}
-
-}  // namespace
-
-int64_t csr_pair_union_nnz(const CsrRef& K) {
-  const int64_t n = K.n;
-  if (n < 2 || n % 2 != 0) return -1;
-  const int64_t np = n / 2;
-  std::vector<int64_t> len(static_cast<size_t>(np));
-  parallel_for(np, [&](int64_t k) { len[k] = merged_len(K, 2 * k, 2 * k + 1); });
-  int64_t s = 0;
-  for (int64_t k = 0; k < np; ++k) s += 2 * len[k];
-  return s;
-}
-
-Csr csr_pair_union(const CsrRef& K) {
-  const int64_t n = K.n;
-  if (n < 2 || n % 2 != 0) bad("row pairing needs an even row count");
-  const int64_t np = n / 2;
-  Csr U;
-  U.n = n;
-  U.ptr.assign(n + 1, 0);
-  std::vector<int64_t> len(static_cast<size_t>(np));
-  parallel_for(np, [&](int64_t k) { len[k] = merged_len(K, 2 * k, 2 * k + 1); });
-  for (int64_t k = 0; k < np; ++k) {
-    U.ptr[2 * k + 1] = U.ptr[2 * k] + len[k];
-    U.ptr[2 * k + 2] = U.ptr[2 * k + 1] + len[k];
-  }
-  U.col.resize(static_cast<size_t>(U.ptr[n]));
-  U.val.resize(static_cast<size_t>(U.ptr[n]));
-  parallel_for(np, [&](int64_t k) {
-    const int64_t a = 2 * k, b = a + 1;
-    int64_t i = K.ptr[a], ie = K.ptr[a + 1], j = K.ptr[b], je = K.ptr[b + 1];
-    int64_t oa = U.ptr[a], ob = U.ptr[b];
-    while (i < ie || j < je) {
-      const int32_t ci = i < ie ? K.col[i] : INT32_MAX, cj = j < je ? K.col[j] : INT32_MAX;
-      const int32_t c = std::min(ci, cj);
-      U.col[oa] = U.col[ob] = c;
-      U.val[oa++] = ci == c ? K.val[i++] : 0.0;
-      U.val[ob++] = cj == c ? K.val[j++] : 0.0;
-    }
-  });
-  return U;
-}
-
 Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C, double dt,
                 int mode, bool host_sell) {
   if (n <= 0) bad("n must be positive");

@@ -80,14 +80,6 @@ Csr normalise_csr(int64_t n, const int64_t* ptr, const int32_t* col,
 // finite and non-zero (checked in parallel). Such a CSR is used in place.
 bool csr_is_normal(int64_t n, const int64_t* ptr, const int32_t* col, const double* val);
 
-// Row pairing of K (rows 2k and 2k+1, the two displacement rows of a node in
-// 2D solids, whose column lists differ only by exact zeros): the entries of
-// K with each pair of rows stored over the union of the pair's columns, with
-// explicit zeros where a row lacks a column. One gather of a neighbour then
-// serves both rows. csr_pair_union_nnz gives the entry count of that form
-// (-1 for an odd n); csr_pair_union builds it (columns sorted).
-int64_t csr_pair_union_nnz(const CsrRef& K);
-Csr csr_pair_union(const CsrRef& K);
 
 // Builds the whole plan; with host_sell = false the SELL-64 operator is left
 // empty (the caller builds it on the device from K and alpha). Throws

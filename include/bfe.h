@@ -76,7 +76,7 @@ typedef struct bfe_info {
   double ebe_check;      /* set-up check: relative step difference to CSR */
   int32_t ebe_classes;   /* neighbour classes of the mesh pattern         */
   int32_t ebe_groups;    /* element groups of the element operator       */
-  int32_t rows_pair;     /* 1 if a row-paired operator was built          */
+  int32_t multi_group;   /* rows per gather group of the last batched run */
   int32_t reserved;
 } bfe_info;
 
@@ -239,13 +239,15 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
 /* rows each wavefront of the multi-source march handles together (1, 2, 4,
  * 8 or 16; default 1): consecutive rows of one 64-row slice, interleaved */
 #define BFE_TUNE_MULTI_ROWS_PER_WAVE 6
-/* 1 (default): when the plan holds a row-paired operator (bfe_info
- * rows_pair: rows 2k and 2k+1 stored over the union of their columns, built
- * when the unions hold at most 0.65 times the entries of K, as for the two
- * rows of a 2D solid node) and a wavefront handles an even number of rows, the
- * multi-source march gathers each neighbour once for both rows; 0: one
- * gather per row. Results agree to rounding. */
-#define BFE_TUNE_MULTI_PAIRED 8
+/* rows per gather group of the multi-source march (1, 2, 4 or 8, at most
+ * the rows per wavefront; 0, the default: automatic). With G > 1 each
+ * aligned group of G rows is stored over the union of its rows' columns
+ * (built on the device at the first batched run that uses it), so one
+ * gather of a neighbour serves G rows: G = 2 for the two rows of a 2D solid
+ * node, G = 4 for two neighbouring nodes. Automatic picks 2 when that saves
+ * at least a third of the gathers, else 1. bfe_info multi_group reports the
+ * group of the last batched run. Results agree to rounding. */
+#define BFE_TUNE_MULTI_GROUP 8
 /* 1: bfe_run captures its whole time loop (step kernels, interface solve,
  * last snapshot) into one hipGraph and launches that, instead of launching
  * step by step; for small meshes whose steps are launch-latency bound.

@@ -215,12 +215,14 @@ def test_batched_tuning_does_not_change_results():
 
 
 @pytest.mark.parametrize("absorbing", [False, True])
-def test_batched_paired_rows_bit_identical(absorbing):
+def test_batched_row_groups_agree(absorbing):
     """2D solid plate (two rows per node whose column lists differ only by
-    exact zeros): the paired gathers of the multi-source kernel agree with one
-    gather per row to rounding, for every even row count per wavefront (and a
-    ragged last slice), and the batched columns match the oracle."""
-    print("case: batched paired rows, absorbing", absorbing, flush=True)
+    exact zeros): the row-grouped gathers of the multi-source kernel (groups
+    of 2, 4 and 8 rows over the union of their columns) agree with one gather
+    per row to rounding, for row counts per wavefront from the group size to
+    16 (and a ragged last slice), and the batched columns match the oracle.
+    The automatic choice pairs the rows."""
+    print("case: batched row groups, absorbing", absorbing, flush=True)
     from bristolfe_v2_amd import _lib
     mod, matls = solid_plate(1.6e-3, 0.9e-3, 0.1e-3)
     if absorbing:
@@ -238,16 +240,18 @@ def test_batched_paired_rows_bit_identical(absorbing):
     hi = np.concatenate([src, np.arange(0, n, 13)])
     plan = PKG.make_plan(K, C, M, dt)
     try:
-        assert plan.info()["rows_pair"] == 1
-        plan.set_tuning(_lib.TUNE_MULTI_PAIRED, 0)
+        plan.set_tuning(_lib.TUNE_MULTI_GROUP, 1)
         ref, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
-        plan.set_tuning(_lib.TUNE_MULTI_PAIRED, 1)
-        for rpw in (2, 4, 8, 16):
-            for rows in (1, 4):
-                plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, rpw)
-                plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, rows)
-                h, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
-                assert rel_l2(h, ref) < 1e-13
+        assert plan.info()["multi_group"] == 1
+        for g in (0, 2, 4, 8):
+            plan.set_tuning(_lib.TUNE_MULTI_GROUP, g)
+            for rpw in sorted({max(g, 2), 8, 16}):
+                for rows in (1, 4):
+                    plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, rpw)
+                    plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, rows)
+                    h, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+                    assert plan.info()["multi_group"] == (g or 2)
+                    assert rel_l2(h, ref) < 1e-13
         for c in (0, S - 1):
             r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, np.arange(n_t) * dt,
                                                         np.array([src[c] + 1]), s, hi + 1)
@@ -257,15 +261,27 @@ def test_batched_paired_rows_bit_identical(absorbing):
         plan.close()
 
 
-def test_rows_pair_detection():
-    """A plan pairs its rows only when that saves at least a third of the
-    gathers: the coupled solid/water model (nodes of 1, 2 and 3 DOFs, so the
-    pairs straddle nodes) does not."""
+def test_row_groups_automatic_choice():
+    """Automatic grouping pairs rows only when that saves at least a third of
+    the gathers: the coupled solid/water model (nodes of 1, 2 and 3 DOFs, so
+    the pairs straddle nodes) keeps one gather per row, and an explicit group
+    is still honoured there (and agrees)."""
+    from bristolfe_v2_amd import _lib
     mod, matls = coupled_block(1.5e-3, 0.75e-3, 0.5e-3, 0.125e-3)
     K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4]})
-    plan = PKG.make_plan(K, C, M, 1e-9)
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    s = PKG.mesh.hann_toneburst(np.arange(100) * dt, 5e6, 3)
+    src = gl[np.nonzero(gl[:, 3] > 0)[0][:8], 3] - 1
+    hi = np.arange(0, K.shape[0], 7)
+    plan = PKG.make_plan(K, C, M, dt)
     try:
-        assert plan.info()["rows_pair"] == 0
+        plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, 8)
+        ref, _ = plan.run_batched(100, src.size, src, np.arange(src.size), s, hi)
+        assert plan.info()["multi_group"] == 1
+        plan.set_tuning(_lib.TUNE_MULTI_GROUP, 4)
+        h, _ = plan.run_batched(100, src.size, src, np.arange(src.size), s, hi)
+        assert plan.info()["multi_group"] == 4
+        assert rel_l2(h, ref) < 1e-13
     finally:
         plan.close()
 
