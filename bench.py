@@ -190,7 +190,7 @@ def bench_fmc(args):
     subdomain models in one 64-column march, against 64 single-source runs."""
     import __graft_entry__ as ge
     ge.load_package()
-    from bristolfe_v2_amd import autotune, configs
+    from bristolfe_v2_amd import _lib, autotune, configs
     from bristolfe_v2_amd.solver import make_plan
     from oracle import v6 as ov6
     kw = {} if args.time_steps is None else {"n_t": args.time_steps}
@@ -204,7 +204,14 @@ def bench_fmc(args):
     res = {}
     for prec in ("double", "single"):
         plan = make_plan(K, C, M, c["dt"], "vel at curent time step", prec)
-        best, table = autotune.tune_batched(plan, S, fd, fs, F, hd)
+        if args.fmc_shape:  # fixed launch shape (short profiled runs): no autotune
+            r_, w_, x_, g_, tile_ = (int(v) for v in args.fmc_shape.split(","))
+            for key, v in ((_lib.TUNE_MULTI_ROWS_PER_BLOCK, r_), (_lib.TUNE_MULTI_ROWS_PER_WAVE, w_),
+                           (_lib.TUNE_XCD_REMAP, x_), (_lib.TUNE_MULTI_GROUP, g_)):
+                plan.set_tuning(key, v)
+            best, table = dict(rows=r_, rpw=w_, remap=x_, group=g_, tile=tile_), []
+        else:
+            best, table = autotune.tune_batched(plan, S, fd, fs, F, hd)
         log(f"{prec}: autotune best {best}")
         walls, loops = [], []
         hb = None
@@ -259,7 +266,7 @@ def bench_fmc(args):
         "roofline": {"bound": "hbm", "achieved": round(bpd * n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bpd * n * S * (n_t - 1) / (res["double"]["loop_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": read_traffic("fmc_batched", "k_step_multi"),
                      "kernel": (f"k_step_multi<double> ({S} sources, {res['double']['best']['rpw']} rows "
                                 f"per wavefront, {res['double']['best']['group']} rows per gather)"),
                      "bytes_per_dof_step_source": round(bpd, 2)},
@@ -345,6 +352,8 @@ def main():
                     help="override the workload's number of time steps (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device copy (counter calibration)")
+    ap.add_argument("--fmc-shape", default=None,
+                    help="fmc_batched: fixed launch shape rows,rpw,remap,group,tile instead of the autotuner")
     ap.add_argument("--fd-elements", type=int, default=50)
     ap.add_argument("--fd-step", type=float, default=1e-5)
     args = ap.parse_args()

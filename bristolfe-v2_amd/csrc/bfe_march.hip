@@ -241,12 +241,14 @@ __device__ __forceinline__ void multi_rows(
     T* __restrict__ hist_prev, const IfaceRows<T>& I) {
   const bool active = lane < S;
   const int64_t sl = row0 >> 6;
-  const int64_t b0 = off[sl] + (row0 & 63);
   const int w = static_cast<int>((off[sl + 1] - off[sl]) >> 6);  // entries per row of the slice
+  const int64_t b0 = off[sl] + (row0 & 63) * static_cast<int64_t>(w);  // rows contiguous
   // The rows' entries are loaded once, one per lane, and broadcast with
   // cross-lane shuffles: the gathers of u1 then issue back to back.
   static_assert(G >= 1 && R % G == 0, "rows per wavefront must be a multiple of the group");
   constexpr int RS = G;  // rows served by one gather
+  constexpr int NG = R / G;  // row groups per wavefront
+  constexpr int U = NG >= 4 ? 2 : NG == 2 ? 4 : 8;
   T acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) acc[r] = T(0);
@@ -259,11 +261,33 @@ __device__ __forceinline__ void multi_rows(
       myv[r] = T(0);
       myc[r] = 0;
       if (lane < cnt) {
-        myv[r] = val[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
-        if (r % RS == 0) myc[r] = col[b0 + r + 64 * static_cast<int64_t>(k0 + lane)];
+        myv[r] = val[b0 + static_cast<int64_t>(r) * w + k0 + lane];
+        if (r % RS == 0) myc[r] = col[b0 + static_cast<int64_t>(r) * w + k0 + lane];
       }
     }
     int k = 0;
+    // U gathers per row group issued back to back (about 8 per wavefront in
+    // flight: the step is bound by gather latency), summed in the same
+    // pairs, in the same order, as the two-entry loop below
+    for (; k + U <= cnt; k += U) {
+#pragma unroll
+      for (int r = 0; r < R; r += RS) {
+        T x[U];
+#pragma unroll
+        for (int e = 0; e < U; ++e) {
+          const int c = lane_bcast(myc[r], k + e);
+          x[e] = T(0);
+          if (active) x[e] = u1[static_cast<int64_t>(c) * S + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < RS; ++j)
+#pragma unroll
+          for (int e = 0; e < U; e += 2) {
+            const T v0 = lane_bcast(myv[r + j], k + e), v1 = lane_bcast(myv[r + j], k + e + 1);
+            acc[r + j] += v0 * x[e] + v1 * x[e + 1];
+          }
+      }
+    }
     for (; k + 2 <= cnt; k += 2) {
 #pragma unroll
       for (int r = 0; r < R; r += RS) {
@@ -1086,7 +1110,7 @@ struct bfe_plan_s {
   int32_t multi_group = 0;  // rows per gather group of k_step_multi (0: automatic)
   int32_t auto_group = 0;   // automatic choice once made (1 or 2)
   int32_t last_group = 0;   // group of the last batched run
-  GroupOp grp[3];           // row-grouped operators for G = 2, 4, 8 (built on use)
+  GroupOp grp[4];           // multi-source operators for G = 1, 2, 4, 8 (built on use)
   int32_t xcd_remap = 1;    // XCD-aware workgroup order
   std::vector<hipEvent_t> tev;  // pairs (start, stop) for sampled launches
   // adjoint support (defined further down)
@@ -1299,12 +1323,14 @@ void build_operators(bfe_plan_s& H, const bfe::CsrRef& K, const std::vector<doub
 }
 
 // ---------------------------------------------------------------------------
-// Row-grouped operators of the multi-source march, built on the device from
-// the SELL-64 operator: each aligned group of G rows (G divides 64, so a
-// group lies in one slice) stored over the union of its rows' columns, in
-// the SELL-64 layout (entry k of row r at off[s] + 64 k + r % 64), explicit
-// zeros where a row lacks a column, padded per slice with value 0. Entries
-// of value 0 in the source (padding) are skipped.
+// Operators of the multi-source march, built on the device from the SELL-64
+// operator: each aligned group of G rows (G = 1, 2, 4 or 8; G divides 64, so
+// a group lies in one slice) stored over the union of its rows' columns,
+// explicit zeros where a row lacks a column, padded per slice to the widest
+// union with value 0. Rows are contiguous within a slice (entry k of row r
+// at off[s] + (r % 64) w_s + k, w_s the slice width), so the entry loads of
+// a wavefront's rows are coalesced. Entries of value 0 in the source
+// (padding) are skipped.
 template <typename T, int G, typename Emit>
 __device__ __forceinline__ int group_merge(int64_t base, int w, const int32_t* __restrict__ col,
                                            const T* __restrict__ val, Emit&& emit) {
@@ -1362,25 +1388,25 @@ __global__ void k_group_fill(int64_t n, int64_t n_slices, const int64_t* __restr
   if (s >= n_slices) return;
   const int r0 = static_cast<int>(g % (64 / G)) * G;
   const int w = static_cast<int>((off[s + 1] - off[s]) >> 6);
-  const int64_t ob = goff[s] + r0;
   const int wg = static_cast<int>((goff[s + 1] - goff[s]) >> 6);
+  const int64_t ob = goff[s] + static_cast<int64_t>(r0) * wg;  // row j at ob + j wg
   const int len = group_merge<T, G>(off[s] + r0, w, col, val, [&](int k, int c, const T* v) {
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      gcol[ob + j + 64 * static_cast<int64_t>(k)] = c;
-      gval[ob + j + 64 * static_cast<int64_t>(k)] = v[j];
+      gcol[ob + static_cast<int64_t>(j) * wg + k] = c;
+      gval[ob + static_cast<int64_t>(j) * wg + k] = v[j];
     }
   });
   for (int k = len; k < wg; ++k)
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-      gcol[ob + j + 64 * static_cast<int64_t>(k)] =
+      gcol[ob + static_cast<int64_t>(j) * wg + k] =
           static_cast<int32_t>(min<int64_t>(s * 64 + r0 + j, n - 1));
-      gval[ob + j + 64 * static_cast<int64_t>(k)] = T(0);
+      gval[ob + static_cast<int64_t>(j) * wg + k] = T(0);
     }
 }
 
-int group_index(int G) { return G == 2 ? 0 : G == 4 ? 1 : 2; }
+int group_index(int G) { return G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : 3; }
 
 template <typename T, int G>
 void build_group_op_g(bfe_plan_s& H, GroupOp& op) {
@@ -1410,12 +1436,13 @@ void build_group_op_g(bfe_plan_s& H, GroupOp& op) {
   op.built = true;
 }
 
-// The grouped operator for G (2, 4 or 8), built on first use.
+// The operator for G (1, 2, 4 or 8), built on first use.
 template <typename T>
 const GroupOp& group_op(bfe_plan_s& H, int G) {
   GroupOp& op = H.grp[group_index(G)];
   if (!op.built) {
-    if (G == 2) build_group_op_g<T, 2>(H, op);
+    if (G == 1) build_group_op_g<T, 1>(H, op);
+    else if (G == 2) build_group_op_g<T, 2>(H, op);
     else if (G == 4) build_group_op_g<T, 4>(H, op);
     else build_group_op_g<T, 8>(H, op);
   }
@@ -1439,7 +1466,7 @@ int resolve_group(bfe_plan_s& H, int rpw) {
     G = H.auto_group;
   }
   G = std::min(G, rpw);
-  if (G > 1) group_op<T>(H, G);
+  group_op<T>(H, G);
   return G;
 }
 
@@ -2220,8 +2247,8 @@ void launch_multi_r(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int64_t n, i
                     const T* u1, const T* u2, T* u, const BlockLists& L, const T* falpha,
                     const T* F, int32_t F_rows, int32_t t, int32_t q, T* hp,
                     const IfaceRows<T>& IR) {
-  const GroupOp* g = G > 1 ? &H.grp[G == 2 ? 0 : G == 4 ? 1 : 2] : nullptr;
-  const DevBuf &o = g ? g->off : H.off, &c = g ? g->col : H.col, &v = g ? g->val : H.val;
+  const GroupOp& g = H.grp[G == 1 ? 0 : G == 2 ? 1 : G == 4 ? 2 : 3];
+  const DevBuf &o = g.off, &c = g.col, &v = g.val;
   hipLaunchKernelGGL((k_step_multi<T, D, R, G>), dim3(nblk), dim3(64 * rpb), 0, H.stream, n, S,
                      nblk, H.xcd_remap, o.as<int64_t>(), c.as<int32_t>(), v.as<T>(),
                      D ? H.beta.as<T>() : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
