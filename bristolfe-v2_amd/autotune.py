@@ -13,9 +13,9 @@ Tunables (none changes results):
           per byte of u)
 Each candidate marches a short probe (a prefix of the real time axis) on the
 real operator; the fastest by HIP-event loop time wins. The search is staged:
-rows x rpw first (XCD order on, automatic group, widest tile), then the
-group, then the XCD order, then the tile width, each at the best shape found
-so far.
+rows x rpw first (XCD order on, automatic group, widest tile), then each
+other group with one or two groups per wavefront over all rows, then the XCD
+order, then the tile width, each at the best shape found so far.
 """
 from __future__ import annotations
 
@@ -79,12 +79,12 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
     widest = min(tiles[0], n_src)
     best = min((measure(r, w, remaps[0], 0, widest) for r, w in itertools.product(rows, rpws)),
                key=lambda d: d["ms_per_step"])
-    for g in groups:
+    for g in groups:  # each group at one and two groups per wavefront
         if g == best["group"]:
             continue
-        w = max(best["rpw"], g)
-        if w in rpws:
-            best = keep(measure(best["rows"], w, remaps[0], g, widest), best)
+        for w in sorted({g, 2 * g} & set(rpws)):
+            for r in rows:
+                best = keep(measure(r, w, remaps[0], g, widest), best)
     for x in remaps[1:]:
         best = keep(measure(best["rows"], best["rpw"], x, best["group"], widest), best)
     for tile in tiles[1:]:
