@@ -494,6 +494,18 @@ def main():
                     "loop_ms": round(gst["loop_ms"], 3), "wall_s": round(gw, 3),
                     "eager_loop_ms": round(float(np.mean(loops)), 3),
                     "identical_to_eager": bool(np.array_equal(hg, hist))}
+    if not use_ebe and n <= 2_000_000:
+        # the same march (field reduction included) as one cooperative launch
+        # of the persistent multi-step kernel
+        plan.set_tuning(9, 1)
+        pl, hp = [], None
+        for _ in range(max(args.steps, 1)):
+            hp, _, _ = march()
+            pl.append(plan.stats()["loop_ms"])
+        plan.set_tuning(9, 0)
+        out["persistent"] = {"gdof_steps_per_s": round(n * steps_done / float(np.mean(pl)) / 1e6, 4),
+                             "loop_ms": round(float(np.mean(pl)), 3),
+                             "identical_to_eager": bool(np.array_equal(hp, hist))}
     if use_ebe:
         out["config"]["operator"] = (f"element-by-element: {op.n_el} elements in "
                                      f"{info['ebe_groups']} group(s), {info['ebe_templates']} "

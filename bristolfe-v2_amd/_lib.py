@@ -27,6 +27,7 @@ TUNE_EBE_KERNEL = 5
 TUNE_MULTI_ROWS_PER_WAVE = 6
 TUNE_GRAPH = 7
 TUNE_MULTI_GROUP = 8
+TUNE_PERSISTENT = 9
 
 # Every symbol the header declares.
 EXPORTS = (

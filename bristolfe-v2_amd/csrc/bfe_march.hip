@@ -15,6 +15,7 @@
 // A wavefront owns one 64-row slice; lane l streams row 64*s+l, so the value
 // and column streams are read fully coalesced (64 consecutive entries per
 // load) and no cross-lane reduction is needed.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -122,16 +123,16 @@ __device__ __forceinline__ void stream_store(T v, T* p) {
 // coupled model; 2-4 % slower on a 40k-DOF plate that fits in cache).
 constexpr int64_t kStreamBytes = 256ll << 20;
 
-template <typename T, int S, bool DAMPED, bool NT = false>
-__global__ __launch_bounds__(kBlock) void k_step(
-    int64_t n, int64_t nblk, const int64_t* __restrict__ off,
+// One row of workgroup block lb (the body of k_step and of the persistent
+// march).
+template <typename T, int S, bool DAMPED, bool NT>
+__device__ __forceinline__ void step_row(
+    int64_t lb, int64_t row, int64_t n, const int64_t* __restrict__ off,
     const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
-    BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
+    const BlockLists& L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
     int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
-    int32_t fld_is_vel, T dt, IfaceRows<T> I, ColPtrs<T, S> store) {
-  const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t row = lb * kRowsSingle + threadIdx.x;
+    int32_t fld_is_vel, T dt, const IfaceRows<T>& I, const ColPtrs<T, S>& store) {
   if (row >= n) return;
   const int64_t sl = row >> 6;
   const int lane = static_cast<int>(row & 63);
@@ -200,6 +201,20 @@ __global__ __launch_bounds__(kBlock) void k_step(
 #pragma unroll
         for (int s = 0; s < S; ++s) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * s] = a[s];
       }
+}
+
+template <typename T, int S, bool DAMPED, bool NT = false>
+__global__ __launch_bounds__(kBlock) void k_step(
+    int64_t n, int64_t nblk, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ col, const T* __restrict__ val, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u,
+    BlockLists L, const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows,
+    int32_t t, int32_t q, T* __restrict__ hist_prev, T* __restrict__ fld_prev,
+    int32_t fld_is_vel, T dt, IfaceRows<T> I, ColPtrs<T, S> store) {
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  step_row<T, S, DAMPED, NT>(lb, lb * kRowsSingle + threadIdx.x, n, off, col, val, beta, u1, u2,
+                             u, L, f_alpha, F, F_rows, t, q, hist_prev, fld_prev, fld_is_vel, dt,
+                             I, store);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,11 +418,10 @@ __global__ __launch_bounds__(512) void k_step_multi(
 constexpr int kIfaceLanes = 16;
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_iface(
-    int32_t nI, const int32_t* __restrict__ iface, const int64_t* __restrict__ xptr,
+__device__ __forceinline__ void iface_lane(
+    int64_t gid, int32_t nI, const int32_t* __restrict__ iface, const int64_t* __restrict__ xptr,
     const int32_t* __restrict__ xcol, const T* __restrict__ xval, const T* __restrict__ wI,
     T* __restrict__ u) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const int64_t k = gid / kIfaceLanes;
   const int j = static_cast<int>(gid % kIfaceLanes);
   T acc = T(0);
@@ -416,6 +430,80 @@ __global__ __launch_bounds__(kBlock) void k_iface(
       acc += xval[p] * wI[xcol[p]];
   for (int o = kIfaceLanes / 2; o > 0; o >>= 1) acc += __shfl_down(acc, o, kIfaceLanes);
   if (k < nI && j == 0) u[iface[k]] = acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_iface(
+    int32_t nI, const int32_t* __restrict__ iface, const int64_t* __restrict__ xptr,
+    const int32_t* __restrict__ xcol, const T* __restrict__ xval, const T* __restrict__ wI,
+    T* __restrict__ u) {
+  iface_lane<T>(static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x, nI, iface, xptr,
+                xcol, xval, wI, u);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent multi-step march: one cooperative launch runs the whole time
+// loop of a single-source CSR march, for meshes small enough that a step is
+// bound by launch latency rather than by HBM. Every step the workgroups
+// sweep the blocks of k_step (workgroup g takes blocks g, g + grid, ..., so
+// the per-workgroup forcing, monitor and interface lists are those of
+// k_step), meet at a grid barrier, and meet again after the interface solve
+// when the plan has one. The loop bounds are kernel arguments, the same for
+// every workgroup, so every workgroup reaches every barrier.
+template <typename T>
+struct PersistentMarch {
+  int64_t n, nblk, t0, n_t;
+  const int64_t* off;
+  const int32_t* col;
+  const T* val;
+  const T* beta;
+  T* ubuf;  // 3 n: u, u1, u2 of the first step, rotated as in the host loop
+  BlockLists L;
+  const T* f_alpha;
+  const T* F;
+  int32_t F_rows, q;
+  T* hist;
+  int32_t field_every;  // 0: no field frames
+  int32_t fld_is_vel;
+  T* fld;
+  T dt;
+  IfaceRows<T> I;
+  int32_t nI;  // interface rows solved after each step (0: none)
+  const int32_t* iface;
+  const int64_t* xptr;
+  const int32_t* xcol;
+  const T* xval;
+};
+
+template <typename T, bool DAMPED, bool NT>
+__global__ __launch_bounds__(kBlock) void k_march_persistent(PersistentMarch<T> a) {
+  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+  const ColPtrs<T, 1> store{{nullptr}};
+  for (int64_t t = a.t0; t < a.n_t; ++t) {
+    // the host loop's rotation (u2, u1, u) <- (u1, u, u2) after each step
+    const int64_t k = (t - a.t0) % 3;
+    T* u = a.ubuf + ((3 - k) % 3) * a.n;
+    const T* u1 = a.ubuf + ((4 - k) % 3) * a.n;
+    const T* u2 = a.ubuf + ((5 - k) % 3) * a.n;
+    const int64_t prev = t - 1;
+    T* hp = (a.q > 0 && prev >= a.t0) ? a.hist + prev * a.q : nullptr;
+    T* fp = (a.field_every > 0 && prev >= a.t0 && prev % a.field_every == 0)
+                ? a.fld + (prev / a.field_every) * a.n
+                : nullptr;
+    for (int64_t lb = blockIdx.x; lb < a.nblk; lb += gridDim.x)
+      step_row<T, 1, DAMPED, NT>(lb, lb * kRowsSingle + threadIdx.x, a.n, a.off, a.col, a.val,
+                                 a.beta, u1, u2, u, a.L, a.f_alpha, a.F, a.F_rows,
+                                 static_cast<int32_t>(t), a.q, hp, fp, a.fld_is_vel, a.dt, a.I,
+                                 store);
+    grid.sync();
+    if (a.nI > 0) {
+      const int64_t work = static_cast<int64_t>(a.nI) * kIfaceLanes;
+      for (int64_t base = static_cast<int64_t>(blockIdx.x) * blockDim.x; base < work;
+           base += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        iface_lane<T>(base + threadIdx.x, a.nI, a.iface, a.xptr, a.xcol, a.xval, a.I.wI, u);
+      grid.sync();
+    }
+  }
 }
 
 // Multi-source: one thread per (k, s); wI and u rows are S-contiguous.
@@ -1125,6 +1213,7 @@ struct bfe_plan_s {
   int32_t ebe_group = 8;             // slots per load group of k_step_ebe (2, 4 or 8)
   int32_t ebe_kernel = 1;            // 1: pattern kernel when the mesh matches one
   int32_t use_graph = 0;             // single-source march captured in a hipGraph
+  int32_t use_persistent = 0;        // single-source CSR march as one cooperative launch
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
@@ -1562,6 +1651,59 @@ void launch_iface(const bfe_plan_s& H, int32_t S, const T* wI, T* u) {
                        H.stream, H.nI, S, H.iface.as<int32_t>(), H.xptr.as<int64_t>(),
                        H.xcol.as<int32_t>(), H.xval.as<T>(), wI, u);
   }
+}
+
+// One cooperative launch of k_march_persistent for steps t0 .. n_t - 1, with
+// as many workgroups as the step has blocks, at most as many as the device
+// holds at once (cooperative launch refuses more).
+template <typename T>
+void launch_persistent(const bfe_plan_s& H, int64_t nblk, int64_t t0, int32_t n_t, T* ubuf,
+                       const BlockLists& L, const T* falpha, const T* F, int32_t F_rows,
+                       int32_t q, T* hist, int32_t field_every, T* fld, int32_t fld_is_vel, T dt,
+                       const IfaceRows<T>& IR) {
+  PersistentMarch<T> a{};
+  a.n = H.n;
+  a.nblk = nblk;
+  a.t0 = t0;
+  a.n_t = n_t;
+  a.off = H.off.as<int64_t>();
+  a.col = H.col.as<int32_t>();
+  a.val = H.val.as<T>();
+  a.beta = H.damped ? H.beta.as<T>() : nullptr;
+  a.ubuf = ubuf;
+  a.L = L;
+  a.f_alpha = falpha;
+  a.F = F;
+  a.F_rows = F_rows;
+  a.q = q;
+  a.hist = hist;
+  a.field_every = field_every;
+  a.fld_is_vel = fld_is_vel;
+  a.fld = fld;
+  a.dt = dt;
+  a.I = IR;
+  a.nI = (H.nI > 0 && H.has_xinv) ? H.nI : 0;
+  a.iface = H.iface.as<int32_t>();
+  a.xptr = H.xptr.as<int64_t>();
+  a.xcol = H.xcol.as<int32_t>();
+  a.xval = H.xval.as<T>();
+  const bool nt = H.nnz_sell * static_cast<int64_t>(sizeof(T) + sizeof(int32_t)) > kStreamBytes;
+  const void* fn =
+      H.damped ? (nt ? reinterpret_cast<const void*>(&k_march_persistent<T, true, true>)
+                     : reinterpret_cast<const void*>(&k_march_persistent<T, true, false>))
+               : (nt ? reinterpret_cast<const void*>(&k_march_persistent<T, false, true>)
+                     : reinterpret_cast<const void*>(&k_march_persistent<T, false, false>));
+  int coop = 0, bpcu = 0, ncu = 0;
+  BFE_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, H.device));
+  if (!coop) throw hip_failure("persistent march: the device has no cooperative launch");
+  BFE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpcu, fn, kBlock, 0));
+  BFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, H.device));
+  const int64_t cap = static_cast<int64_t>(bpcu) * ncu;
+  if (cap < 1) throw hip_failure("persistent march: no co-resident workgroups");
+  const int64_t grid = std::min(nblk, cap);
+  void* args[] = {&a};
+  BFE_HIP(hipLaunchCooperativeKernel(fn, dim3(static_cast<uint32_t>(grid)), dim3(kBlock), args, 0,
+                                     H.stream));
 }
 
 void validate_run(int64_t n, int32_t n_t, const int32_t* frc_idx, int32_t p, const double* frcs,
@@ -2007,10 +2149,13 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     rec->cur = 0;
   }
   int64_t launches = 0;
-  // graph mode: the whole time loop is captured into one hipGraph and then
-  // launched (plain marches only; no per-kernel sampling)
-  const bool graph = H.use_graph != 0 && rec == nullptr;
-  KernelSampler sampler(H, t0, graph ? 0 : n_t - t0);
+  // persistent mode: one cooperative launch runs the loop (CSR operator,
+  // plain marches); graph mode: the whole time loop is captured into one
+  // hipGraph and then launched (plain marches only). Neither samples
+  // per-step kernels.
+  const bool persistent = H.use_persistent != 0 && E == nullptr && rec == nullptr;
+  const bool graph = H.use_graph != 0 && rec == nullptr && !persistent;
+  KernelSampler sampler(H, t0, (graph || persistent) ? 0 : n_t - t0);
   GraphCapture cap;
   if (graph) {
     BFE_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -2018,7 +2163,16 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
   } else {
     BFE_HIP(hipEventRecord(H.ev0, st));
   }
-  for (int64_t t = t0; t < n_t; ++t) {
+  if (persistent) {
+    launch_persistent<T>(H, nblk, t0, n_t, ubuf.as<T>(), L, falpha, F, frc_rows, q, hist,
+                         n_frames > 0 ? field_every : 0, fld, field_is_vel, dtT, IR);
+    ++launches;
+    const int64_t k = (n_t - t0) % 3;  // the rotation after the last step
+    u = ubuf.as<T>() + ((3 - k) % 3) * n;
+    u1 = ubuf.as<T>() + ((4 - k) % 3) * n;
+    u2 = ubuf.as<T>() + ((5 - k) % 3) * n;
+  }
+  for (int64_t t = persistent ? n_t : t0; t < n_t; ++t) {
     const int64_t prev = t - 1;
     T* hp = (q > 0 && prev >= t0) ? hist + prev * q : nullptr;
     T* fp = (n_frames > 0 && prev >= t0 && prev % field_every == 0)
@@ -2927,6 +3081,9 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         break;
       case BFE_TUNE_GRAPH:
         h->use_graph = value ? 1 : 0;
+        break;
+      case BFE_TUNE_PERSISTENT:
+        h->use_persistent = value ? 1 : 0;
         break;
       case BFE_TUNE_EBE_KERNEL:
         h->ebe_kernel = value ? 1 : 0;

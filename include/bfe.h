@@ -254,6 +254,12 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * Plain marches only (not bfe_run_checkpointed); no kernel sampling.
  * Default 0. */
 #define BFE_TUNE_GRAPH 7
+/* 1: bfe_run marches the CSR operator with one cooperative launch of a
+ * persistent kernel that runs every time step (grid barrier between steps,
+ * and after the interface solve), for meshes whose steps are launch-latency
+ * bound. Plain marches on the CSR operator only (not with an element operator
+ * in use, not bfe_run_checkpointed); no kernel sampling. Default 0. */
+#define BFE_TUNE_PERSISTENT 9
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3

@@ -286,6 +286,58 @@ def test_row_groups_automatic_choice():
         plan.close()
 
 
+@pytest.mark.parametrize("model", ["coupled", "plate"])
+@pytest.mark.parametrize("mode", MODES)
+def test_persistent_march_matches_eager(model, mode):
+    """The persistent multi-step kernel (one cooperative launch for the whole
+    loop, grid barriers between steps and after the interface solve) gives
+    the eager march's bits: the coupled model (interface rows, absorbing
+    layer) and an absorbing plate, forcing with leading zeros, monitors and
+    velocity field frames; and the oracle's histories."""
+    print("case: persistent march", model, mode, flush=True)
+    from bristolfe_v2_amd import _lib
+    if model == "coupled":
+        mod, matls = coupled_block(2e-3, 1.0e-3, 0.75e-3, 0.125e-3)
+        K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
+        nd = PKG.mesh.fn_find_node_at_point(mod["nds"], [1e-3, 1.5e-3])
+        fi = np.array([gl[nd - 1, 3] - 1, gl[3, 1] - 1])
+    else:
+        mod, matls = solid_plate(2e-3, 1e-3, 0.1e-3)
+        inner = np.array([[0.3e-3, 0.3e-3], [1.7e-3, 0.3e-3], [1.7e-3, 5e-3], [0.3e-3, 5e-3]])
+        mod = PKG.mesh.fn_add_absorbing_layer(mod, inner, 0.3e-3)
+        K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2], "damping_power_law": 2})
+        nd = PKG.mesh.fn_find_node_at_point(mod["nds"], [1e-3, 1e-3])
+        fi = np.array([gl[nd - 1, 1] - 1, gl[nd - 1, 0] - 1])
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    n_t = 300
+    t = np.arange(n_t) * dt
+    F = np.vstack([PKG.mesh.hann_toneburst(t - 25 * dt, 5e6, 3),
+                   0.5 * PKG.mesh.hann_toneburst(t - 40 * dt, 4e6, 2)])
+    hi = np.unique(np.concatenate([fi, np.arange(0, K.shape[0], 29)]))
+    plan = PKG.make_plan(K, C, M, dt, mode)
+    try:
+        assert (plan.info()["n_interface"] > 0) == (model == "coupled")
+        h0, f0, ti0 = plan.run(n_t, fi, F, hi, 50, 1)
+        plan.set_tuning(_lib.TUNE_PERSISTENT, 1)
+        h1, f1, ti1 = plan.run(n_t, fi, F, hi, 50, 1)
+        assert plan.stats()["launches"] <= 2  # the march and the last snapshot
+        assert ti1 == ti0 > 20
+        np.testing.assert_array_equal(h1, h0)
+        np.testing.assert_array_equal(f1, f0)
+        # a second persistent run on the same plan (buffers and lists rebuilt)
+        h2, _, _ = plan.run(n_t, fi[:1], F[:1], hi, 50, 1)
+        plan.set_tuning(_lib.TUNE_PERSISTENT, 0)
+        h3, _, _ = plan.run(n_t, fi[:1], F[:1], hi, 50, 1)
+        np.testing.assert_array_equal(h2, h3)
+    finally:
+        plan.close()
+    r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, F, hi + 1, 50, "KE", mode)
+    assert rti == ti1
+    assert np.linalg.norm(r) > 0
+    assert rel_l2(h1, r) < TOL
+    assert rel_l2(f1, rf) < TOL
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_cube_3d_tets_parity(mode):
     """C3D4 cube with side-face absorbing layers, 3 x 3-node patch source,
