@@ -256,9 +256,11 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
 #define BFE_TUNE_GRAPH 7
 /* 1: bfe_run marches the CSR operator with one cooperative launch of a
  * persistent kernel that runs every time step (grid barrier between steps,
- * and after the interface solve), for meshes whose steps are launch-latency
- * bound. Plain marches on the CSR operator only (not with an element operator
- * in use, not bfe_run_checkpointed); no kernel sampling. Default 0. */
+ * and after the interface solve). Results are identical to the step-by-step
+ * march. On MI355X a grid barrier (about 16 us) costs more than a kernel
+ * boundary, so BFE_TUNE_GRAPH is the faster choice for small meshes. Plain
+ * marches on the CSR operator only (not with an element operator in use, not
+ * bfe_run_checkpointed); no kernel sampling. Default 0. */
 #define BFE_TUNE_PERSISTENT 9
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
