@@ -144,8 +144,9 @@ def test_driver_end_to_end():
     assert rel_l2(r["fld"], ke) < TOL
 
 
-def test_batched_matches_single_runs():
-    print("case: batched", flush=True)
+@pytest.mark.parametrize("mode", MODES)
+def test_batched_matches_single_runs(mode):
+    print("case: batched", mode, flush=True)
     mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3)
     K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
     dt = PKG.mesh.stable_time_step(K, M.diagonal())
@@ -155,7 +156,7 @@ def test_batched_matches_single_runs():
     top = np.nonzero(gl[:, 3] > 0)[0][:12] + 1
     src = gl[top - 1, 3]
     hi = np.concatenate([src, np.arange(1, K.shape[0] + 1, 31)])
-    plan = PKG.make_plan(K, C, M, dt)
+    plan = PKG.make_plan(K, C, M, dt, mode)
     try:
         hb, ti = plan.run_batched(n_t, src.size, src - 1, np.arange(src.size), s, hi - 1)
         for k in range(src.size):
@@ -261,19 +262,20 @@ def test_batched_row_groups_agree(absorbing):
         plan.close()
 
 
-def test_row_groups_automatic_choice():
+@pytest.mark.parametrize("mode", MODES)
+def test_row_groups_automatic_choice(mode):
     """Automatic grouping pairs rows only when that saves at least a third of
     the gathers: the coupled solid/water model (nodes of 1, 2 and 3 DOFs, so
     the pairs straddle nodes) keeps one gather per row, and an explicit group
-    is still honoured there (and agrees)."""
+    is still honoured there (and agrees, interface rows included)."""
     from bristolfe_v2_amd import _lib
     mod, matls = coupled_block(1.5e-3, 0.75e-3, 0.5e-3, 0.125e-3)
-    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4]})
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
     dt = PKG.mesh.stable_time_step(K, M.diagonal())
     s = PKG.mesh.hann_toneburst(np.arange(100) * dt, 5e6, 3)
     src = gl[np.nonzero(gl[:, 3] > 0)[0][:8], 3] - 1
     hi = np.arange(0, K.shape[0], 7)
-    plan = PKG.make_plan(K, C, M, dt)
+    plan = PKG.make_plan(K, C, M, dt, mode)
     try:
         plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, 8)
         ref, _ = plan.run_batched(100, src.size, src, np.arange(src.size), s, hi)
