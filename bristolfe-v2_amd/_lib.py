@@ -52,7 +52,7 @@ class BfeInfo(C.Structure):
         ("device", C.c_int32), ("device_bytes", C.c_int64), ("operator_kind", C.c_int32),
         ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
         ("ebe_elements", C.c_int64), ("ebe_check", C.c_double), ("ebe_classes", C.c_int32),
-        ("ebe_groups", C.c_int32), ("multi_group", C.c_int32), ("reserved", C.c_int32),
+        ("ebe_groups", C.c_int32), ("multi_group", C.c_int32), ("ebe_sparse", C.c_int32),
     ]
 
 

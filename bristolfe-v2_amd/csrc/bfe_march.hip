@@ -743,16 +743,37 @@ struct PatKuhn24 {  // cubic cells split into 6 tetrahedra around a main diagona
       {4, 5, 7, 11}, {4, 6, 7, 11}, {5, 7, 8, 12}, {5, 7, 11, 12}, {6, 7, 9, 13},
       {6, 7, 11, 13}, {7, 8, 10, 14}, {7, 8, 12, 14}, {7, 9, 10, 14}, {7, 9, 13, 14},
       {7, 11, 12, 14}, {7, 11, 13, 14}};
+  // Entries of slot k's template rows (row c, bit b*3 + d: canonical vertex
+  // b, component d) that are not structurally zero: on axis-aligned cells
+  // with an isotropic material, 420 of the 864 products of a node vanish
+  // (the shape-function gradients of these tetrahedra have zero
+  // components). A group takes the product over these entries only when
+  // every other entry of its slot rows is zero or the rounding residue of
+  // a cancellation in B^T D B (at most 1e-14 of the largest entry; about
+  // 1e-17 on the cube), checked at set-up: the sums then agree with the
+  // full product to rounding.
+  static constexpr uint16_t nz[NS][3] = {
+      {0x264, 0x5a0, 0x999}, {0x252, 0x5a9, 0x990}, {0x360, 0x4a4, 0x95a},
+      {0x372, 0x489, 0x948}, {0x2d0, 0x4ec, 0x912}, {0x2f4, 0x4c8, 0x909},
+      {0xb7a, 0x4bd, 0xb7a}, {0x6fc, 0x6fc, 0x93b}, {0x27e, 0xdb9, 0xdb9},
+      {0x6fc, 0x6fc, 0x93b}, {0x5ed, 0xbd2, 0x5ed}, {0xdc9, 0x3f6, 0x3f6},
+      {0x27e, 0xdb9, 0xdb9}, {0xb7a, 0x4bd, 0xb7a}, {0x9db, 0x9db, 0x7e4},
+      {0xdc9, 0x3f6, 0x3f6}, {0x9db, 0x9db, 0x7e4}, {0x5ed, 0xbd2, 0x5ed},
+      {0x999, 0x05a, 0x264}, {0x5a9, 0x252, 0x06c}, {0x099, 0x95a, 0x4a4},
+      {0x489, 0x372, 0x0b4}, {0x129, 0x912, 0x4ec}, {0x909, 0x132, 0x2f4}};
+  static constexpr bool nzbit(int k, int c, int col) { return (nz[k][c] >> col) & 1u; }
 };
 struct PatTriRect {  // squares split into two right triangles
   static constexpr int ID = 2, NPE = 3, NS = 6, NC = 7, SELF = 3;
   static constexpr int8_t cls[NS][NPE] = {{0, 1, 3}, {0, 2, 3}, {1, 3, 4},
                                           {2, 3, 5}, {3, 4, 6}, {3, 5, 6}};
+  static constexpr bool nzbit(int, int, int) { return true; }
 };
 struct PatTriIso {  // near-equilateral triangles
   static constexpr int ID = 3, NPE = 3, NS = 6, NC = 7, SELF = 2;
   static constexpr int8_t cls[NS][NPE] = {{0, 1, 2}, {0, 2, 3}, {1, 2, 4},
                                           {2, 3, 6}, {2, 4, 5}, {2, 5, 6}};
+  static constexpr bool nzbit(int, int, int) { return true; }
 };
 
 // Class offsets (w - v) of a pattern's neighbours on regular nodes.
@@ -792,7 +813,7 @@ struct AnchorSlots {
 // non-temporal streams (on the 2D coupled model 103 -> 92 us per step; on
 // the 3D tet mesh the same was 3 % slower). Template rows (canonical vertex
 // order) come through the scalar cache.
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool REG>
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool REG, bool SP = false>
 __device__ __forceinline__ void ebe_pat_node(
     int64_t v, int64_t lb, int64_t n_nodes, int64_t n, int64_t r0, uint32_t pm, uint32_t sm,
     const ClassOffsets<PT::NC>& co, const AnchorSlots<PT::NS>& an,
@@ -884,7 +905,9 @@ __device__ __forceinline__ void ebe_pat_node(
 #pragma unroll
       for (int d = 0; d < DPN; ++d)
 #pragma unroll
-        for (int c = 0; c < DPN; ++c) y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
+        for (int c = 0; c < DPN; ++c)
+          if (!SP || PT::nzbit(k, c, b * DPN + d))  // folds at compile time
+            y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
     const uint32_t ob = (pm >> k) & 1u;
 #pragma unroll
     for (int c = 0; c < DPN; ++c) acc[c] += pick_bits(s * y[c], T(0), ob);
@@ -894,7 +917,7 @@ __device__ __forceinline__ void ebe_pat_node(
                                      coef_per_node, (pm >> 30) & 1u);
 }
 
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR>
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_step_ebe_pat(
     int64_t n_nodes, int64_t nblk, int64_t n, const int32_t* __restrict__ dof0,
     ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an, const uint32_t* __restrict__ pmask,
@@ -916,7 +939,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   // (the split pays on the anchored triangle groups; on the 24-slot tet
   // pattern the second copy costs more scalar registers than it saves)
   if (!ANCHOR || __any(static_cast<int>(pm >> 31)))
-    ebe_pat_node<T, DAMPED, PT, DPN, ANCHOR, false>(
+    ebe_pat_node<T, DAMPED, PT, DPN, ANCHOR, false, SP>(
         v, lb, n_nodes, n, r0, pm, sm, co, an, nbr, adj, el_scale, anchor, srows, stencil, alpha,
         beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, fld_prev, fld_is_vel, dt, I,
         coef_per_node);
@@ -1115,6 +1138,7 @@ struct EbeGroup {
   int32_t pattern = 0;       // mesh pattern of k_step_ebe_pat, 0: none
   bool uniform = false;      // node j's first DOF is j*dpn (no DOF table read)
   bool smask_all = false;    // most occupied slots scaled (anchor table, no smask)
+  bool sparse = false;       // slot rows zero wherever the pattern's nz mask is (PatKuhn24)
   int32_t coef_node = 0;     // alpha/beta per group node (alpha_n, beta_n)
   std::vector<int64_t> doff; // first-DOF offset of each neighbour class on regular nodes
   std::vector<int32_t> anc_noff, anc_row;  // anchor table column offset and row per slot
@@ -1861,13 +1885,20 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int
       return;
     }
   }
-  hipLaunchKernelGGL((k_step_ebe_pat<T, D, PT, DPN, false>), dim3(nblk), dim3(kBlock), 0,
-                       H.stream, Gr.n_nodes, nblk, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(),
-                       co, an, Gr.pmask.as<uint32_t>(), sm, Gr.nbr.as<int32_t>(),
-                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(), static_cast<const T*>(nullptr),
-                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha_of<T>(Gr),
-                       D ? E.beta_of<T>(Gr) : nullptr, u1, u2, u, L, falpha, F, F_rows, t, q, hp,
-                       fp, fld_is_vel, dt, IR, Gr.coef_node);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, H.stream, Gr.n_nodes, nblk, H.n,
+                       Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), co, an,
+                       Gr.pmask.as<uint32_t>(), sm, Gr.nbr.as<int32_t>(), Gr.adj.as<int32_t>(),
+                       Gr.scale.as<T>(), static_cast<const T*>(nullptr), Gr.srows_c.as<T>(),
+                       Gr.stencil.as<T>(), E.alpha_of<T>(Gr), D ? E.beta_of<T>(Gr) : nullptr, u1,
+                       u2, u, L, falpha, F, F_rows, t, q, hp, fp, fld_is_vel, dt, IR,
+                       Gr.coef_node);
+  };
+  // structurally sparse slot rows (checked at set-up): zero products skipped
+  if (std::is_same<PT, PatKuhn24>::value && Gr.sparse)
+    go(k_step_ebe_pat<T, D, PT, DPN, false, true>);
+  else
+    go(k_step_ebe_pat<T, D, PT, DPN, false, false>);
 }
 
 // Whether a kernel exists for the group's shape (generic) or pattern.
@@ -2727,6 +2758,21 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
           else Gr.anchor = upload(cast_vec<float>(anc));
         }
       }
+      // structurally sparse slot rows: every entry outside the pattern's
+      // nonzero mask is 0 or cancellation residue (<= 1e-14 of the largest)
+      if (Gr.pattern == PatKuhn24::ID && dpn == 3) {
+        bool ok = Lay.n_slot == PatKuhn24::NS;
+        const size_t m = static_cast<size_t>(PatKuhn24::NS) * dpn * KE;
+        double big = 0.0;
+        for (size_t i = 0; ok && i < m; ++i) big = std::max(big, std::fabs(Lay.srows_c[i]));
+        for (int32_t k = 0; k < Lay.n_slot && ok; ++k)
+          for (int c = 0; c < 3 && ok; ++c)
+            for (int col = 0; col < KE && ok; ++col)
+              ok = PatKuhn24::nzbit(k, c, col) ||
+                   std::fabs(Lay.srows_c[(static_cast<size_t>(k) * dpn + c) * KE + col]) <=
+                       1e-14 * big;
+        Gr.sparse = ok;
+      }
       // summed stencil: B_j = sum over (slot, vertex) of class j
       stencil.assign(static_cast<size_t>(nc) * dpn * dpn, 0.0);
       for (int32_t k = 0; k < Lay.n_slot; ++k)
@@ -3136,6 +3182,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
       I.ebe_pattern = E.groups[0].pattern;
       I.ebe_classes = E.groups[0].n_cls;
       I.ebe_groups = static_cast<int32_t>(E.groups.size());
+      for (const auto& g : E.groups) I.ebe_sparse += g.sparse;
     }
     I.multi_group = h->last_group;
     *info = I;

@@ -77,7 +77,8 @@ typedef struct bfe_info {
   int32_t ebe_classes;   /* neighbour classes of the mesh pattern         */
   int32_t ebe_groups;    /* element groups of the element operator       */
   int32_t multi_group;   /* rows per gather group of the last batched run */
-  int32_t reserved;
+  int32_t ebe_sparse;    /* element groups on a structurally sparse product
+                            (zero template entries skipped; equal to rounding) */
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events

@@ -80,6 +80,8 @@ def test_ebe_parity(name, mode, kernel):
         assert info["ebe_pattern"] == PATTERN[name]
         assert info["ebe_check"] < 1e-11
         assert info["ebe_groups"] == (2 if name.startswith("coupled") else 1)
+        # isotropic tets on axis-aligned cells: the structurally sparse product
+        assert info["ebe_sparse"] == (1 if name == "cube" else 0)
         if name.startswith("coupled"):
             assert info["n_interface"] > 0
         plan.set_tuning(_lib.TUNE_EBE_KERNEL, 1 if kernel == "pattern" else 0)
