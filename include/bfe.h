@@ -144,9 +144,9 @@ bfe_status bfe_run(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
                    int32_t* ti_start_out);
 
 /*
- * Per-element field output on the device (SURVEY next row: the reduction of
+ * Per-element field output on the device: the reduction of
  * code/internal/fn_get_field_output.m:64-79, called at
- * code/fn_BristolFE_v2.m:145, moved next to the march).
+ * code/fn_BristolFE_v2.m:145, moved next to the march.
  *
  * bfe_set_field_elements — the mesh tables of the reduction:
  *   node_dof (n_nodes x max_df)  gl_lookup - 1: global DOF of (node, dof), -1 if absent
