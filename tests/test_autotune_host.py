@@ -1,0 +1,79 @@
+"""The batched-march autotuner's staged search (host logic) against a
+synthetic plan whose step time is a known function of the launch shape: the
+search finds the optimum within its stages, leaves the plan tuned to it, and
+never asks for a row group larger than the rows per wavefront."""
+import numpy as np
+
+from bristolfe_v2_amd import _lib, autotune
+
+KEYS = {_lib.TUNE_MULTI_ROWS_PER_BLOCK: "rows", _lib.TUNE_MULTI_ROWS_PER_WAVE: "rpw",
+        _lib.TUNE_XCD_REMAP: "remap", _lib.TUNE_MULTI_GROUP: "group"}
+
+
+class SyntheticPlan:
+    """Stands in for _lib.Plan: run_batched returns zeros, stats() a loop
+    time from cost(shape, columns); group 0 means automatic (pairs)."""
+
+    def __init__(self, cost):
+        self.cost = cost
+        self.shape = {"rows": 4, "rpw": 1, "remap": 1, "group": 0}
+        self.requests = []
+        self._ms = 0.0
+
+    def set_tuning(self, key, value):
+        self.shape[KEYS[key]] = value
+
+    def used_group(self):
+        g = self.shape["group"] or 2
+        return min(g, self.shape["rpw"])
+
+    def info(self):
+        return {"multi_group": self.used_group()}
+
+    def run_batched(self, n_t, S, frc_dof, frc_src, frcs, hist_dof):
+        shape = dict(self.shape, group=self.used_group(), columns=S)
+        self.requests.append(shape)
+        self._ms = self.cost(shape) * (n_t - 1)
+        return np.zeros((len(hist_dof), S, n_t)), 1
+
+    def stats(self):
+        return {"loop_ms": self._ms}
+
+
+def _cost(s):
+    # optimum: 2 wavefronts per workgroup, 4 rows per wavefront, groups of 4,
+    # plain order, all 64 columns in one pass
+    c = 1.0 + 0.1 * abs(s["rows"] - 2) + 0.05 * abs(np.log2(s["rpw"]) - 2)
+    c -= {1: 0.0, 2: 0.2, 4: 0.3, 8: 0.25}[s["group"]]
+    c += 0.02 * s["remap"]
+    return c * (64 / s["columns"])
+
+
+def test_staged_search_finds_the_optimum_and_tunes_the_plan():
+    plan = SyntheticPlan(_cost)
+    fd = np.arange(64)
+    best, table = autotune.tune_batched(plan, 64, fd, np.arange(64), np.ones((1, 400)), fd,
+                                        probe_steps=20, repeats=1)
+    assert (best["rows"], best["rpw"], best["group"], best["remap"], best["tile"]) == (2, 4, 4, 0, 64)
+    assert plan.shape == {"rows": 2, "rpw": 4, "remap": 0, "group": 4}
+    assert min(r["ms_per_step"] for r in table) == best["ms_per_step"]
+    # every request kept the group within the rows per wavefront
+    assert all(r["group"] <= r["rpw"] for r in plan.requests)
+    # the 32-column tile was tried, in two passes of 32 columns
+    assert any(r["columns"] == 32 for r in plan.requests)
+
+
+def test_probe_starts_at_the_first_forced_sample():
+    seen = []
+
+    class P(SyntheticPlan):
+        def run_batched(self, n_t, S, frc_dof, frc_src, frcs, hist_dof):
+            seen.append((n_t, np.asarray(frcs).shape[1]))
+            return super().run_batched(n_t, S, frc_dof, frc_src, frcs, hist_dof)
+
+    F = np.zeros((1, 500))
+    F[0, 37:] = 1.0
+    fd = np.arange(8)
+    autotune.tune_batched(P(_cost), 8, fd, np.arange(8), F, fd, probe_steps=20, repeats=1,
+                          rows=(1, 2), rpws=(1, 2), tiles=(64,))
+    assert all(n == 37 + 20 for n, _ in seen)
