@@ -80,7 +80,6 @@ Csr normalise_csr(int64_t n, const int64_t* ptr, const int32_t* col,
 // finite and non-zero (checked in parallel). Such a CSR is used in place.
 bool csr_is_normal(int64_t n, const int64_t* ptr, const int32_t* col, const double* val);
 
-
 // Builds the whole plan; with host_sell = false the SELL-64 operator is left
 // empty (the caller builds it on the device from K and alpha). Throws
 // std::invalid_argument (bad input) or std::domain_error (numerical
