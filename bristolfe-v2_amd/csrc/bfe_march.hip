@@ -1138,7 +1138,7 @@ struct EbeGroup {
   int32_t pattern = 0;       // mesh pattern of k_step_ebe_pat, 0: none
   bool uniform = false;      // node j's first DOF is j*dpn (no DOF table read)
   bool smask_all = false;    // most occupied slots scaled (anchor table, no smask)
-  bool sparse = false;       // slot rows zero wherever the pattern's nz mask is (PatKuhn24)
+  bool sparse = false;       // slot rows zero (to rounding residue) outside PatKuhn24::nz
   int32_t coef_node = 0;     // alpha/beta per group node (alpha_n, beta_n)
   std::vector<int64_t> doff; // first-DOF offset of each neighbour class on regular nodes
   std::vector<int32_t> anc_noff, anc_row;  // anchor table column offset and row per slot
@@ -1190,7 +1190,8 @@ namespace bfe {
 void set_subdomain_error(const std::string& msg) { g_last_error = msg; }
 }  // namespace bfe
 
-// A row-grouped SELL-64 operator of the multi-source march (see group_op).
+// An operator of the multi-source march: row groups over column unions, rows
+// contiguous within each 64-row slice (see group_op).
 struct GroupOp {
   DevBuf off, col, val;
   int64_t gathers = 0;  // union entries summed over the groups
