@@ -7,7 +7,7 @@ Tunables (none changes results):
           8, 16
   remap   XCD-aware workgroup order on/off
   group   rows per gather group (one u1 gather serves the group's rows over
-          the union of their columns): automatic, 1, 4, 8
+          the union of their columns): automatic, 1, 2, 4, 8
   tile    source columns marched together per pass: 64 or 32 (fewer columns
           per pass re-read the operator more often but keep more lanes busy
           per byte of u)
@@ -49,7 +49,7 @@ def run_tiled(plan, n_t, n_src, frc_dof, frc_src, frcs, hist_dof, tile):
 
 def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
                  rows=(1, 2, 4, 8), rpws=(1, 2, 4, 8, 16), remaps=(1, 0), tiles=(64, 32),
-                 groups=(1, 4, 8), repeats=2):
+                 groups=(1, 2, 4, 8), repeats=2):
     """Returns (best, table): best = dict(rows, rpw, remap, group, tile,
     ms_per_step); group is the one the library used."""
     frcs = np.atleast_2d(frcs)

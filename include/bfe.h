@@ -230,15 +230,15 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
 
 /*
  * bfe_set_tuning — launch shapes and operator choice (no counterpart in the
- * reference): wavefronts per workgroup of the one-wavefront-per-row
- * multi-source kernel (1, 2, 4 or 8; default 4), the XCD-aware workgroup
- * order (default on), and the keys below. Results do not depend on any of
+ * reference): wavefronts per workgroup of the multi-source kernel (1, 2, 4
+ * or 8; default 2), the XCD-aware workgroup order (default on), and the keys
+ * below. Results do not depend on any of
  * them beyond rounding; the autotuner picks the multi-source shape by timing.
  */
 #define BFE_TUNE_MULTI_ROWS_PER_BLOCK 1
 #define BFE_TUNE_XCD_REMAP 2
 /* rows each wavefront of the multi-source march handles together (1, 2, 4,
- * 8 or 16; default 1): consecutive rows of one 64-row slice, interleaved */
+ * 8 or 16; default 4): consecutive rows of one 64-row slice, interleaved */
 #define BFE_TUNE_MULTI_ROWS_PER_WAVE 6
 /* rows per gather group of the multi-source march (1, 2, 4 or 8, at most
  * the rows per wavefront; 0, the default: automatic). With G > 1 each
@@ -246,8 +246,10 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * (built on the device at the first batched run that uses it), so one
  * gather of a neighbour serves G rows: G = 2 for the two rows of a 2D solid
  * node, G = 4 for two neighbouring nodes. Automatic picks 2 when that saves
- * at least a third of the gathers, else 1. bfe_info multi_group reports the
- * group of the last batched run. Results agree to rounding. */
+ * at least a third of the gathers, 4 when that saves a further fifth of
+ * those, else 1 (never more than the rows per wavefront). bfe_info
+ * multi_group reports the group of the last batched run. Results agree to
+ * rounding. */
 #define BFE_TUNE_MULTI_GROUP 8
 /* 1: bfe_run captures its whole time loop (step kernels, interface solve,
  * last snapshot) into one hipGraph and launches that, instead of launching
