@@ -1030,6 +1030,9 @@ __global__ __launch_bounds__(kBlock) void k_grad(
   for (int i = 0; i < KMAX * KMAX; ++i) Ke[i] = el_K[e * KMAX * KMAX + i];
   const T w0 = el_w[3 * e], w1 = el_w[3 * e + 1], w2 = el_w[3 * e + 2];
   double sk = 0.0, sm = 0.0;
+  // two steps' gathers in flight (adjoint 1397 -> 1379 ms on 361k DOF x 16
+  // transmitters; four spilled): the sums keep their order
+#pragma unroll 2
   for (int32_t t = 0; t < Ls; ++t) {
     T ut[KMAX], lt[KMAX];
 #pragma unroll
