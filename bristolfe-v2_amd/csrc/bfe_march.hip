@@ -1031,7 +1031,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(
   const T w0 = el_w[3 * e], w1 = el_w[3 * e + 1], w2 = el_w[3 * e + 2];
   double sk = 0.0, sm = 0.0;
   // two steps' gathers in flight (adjoint 1397 -> 1379 ms on 361k DOF x 16
-  // transmitters; four spilled): the sums keep their order
+  // transmitters; four took all 256 VGPRs and ran slower): the sums keep
+  // their order
 #pragma unroll 2
   for (int32_t t = 0; t < Ls; ++t) {
     T ut[KMAX], lt[KMAX];
