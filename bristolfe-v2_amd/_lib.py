@@ -28,6 +28,7 @@ TUNE_MULTI_ROWS_PER_WAVE = 6
 TUNE_GRAPH = 7
 TUNE_MULTI_GROUP = 8
 TUNE_PERSISTENT = 9
+TUNE_PERSISTENT_GRID = 10
 
 # Every symbol the header declares.
 EXPORTS = (

@@ -29,10 +29,15 @@ from . import _lib
 
 def run_tiled(plan, n_t, n_src, frc_dof, frc_src, frcs, hist_dof, tile):
     """Marches n_src columns in passes of `tile`; returns hist (q, n_src, n_t)
-    and the summed loop time (ms)."""
+    and the summed loop time (ms). frcs holds one signal row for every
+    forcing entry, or one row broadcast to all of them; each pass takes the
+    rows of its own entries."""
     q = np.asarray(hist_dof).size
     frc_dof = np.asarray(frc_dof)
     frc_src = np.asarray(frc_src)
+    frcs = np.atleast_2d(frcs)
+    if frcs.shape[0] not in (1, frc_dof.size):
+        raise ValueError("forcing functions must have 1 or p rows")
     if tile >= n_src:  # one pass: the library's output is the result
         h, _ = plan.run_batched(n_t, n_src, frc_dof, frc_src, frcs, hist_dof)
         return h, plan.stats()["loop_ms"]
@@ -41,7 +46,8 @@ def run_tiled(plan, n_t, n_src, frc_dof, frc_src, frcs, hist_dof, tile):
     for c0 in range(0, n_src, tile):
         c1 = min(c0 + tile, n_src)
         sel = (frc_src >= c0) & (frc_src < c1)
-        h, _ = plan.run_batched(n_t, c1 - c0, frc_dof[sel], frc_src[sel] - c0, frcs, hist_dof)
+        rows = frcs if frcs.shape[0] == 1 else frcs[sel]
+        h, _ = plan.run_batched(n_t, c1 - c0, frc_dof[sel], frc_src[sel] - c0, rows, hist_dof)
         out[:, c0:c1, :] = h
         ms += plan.stats()["loop_ms"]
     return out, ms
@@ -68,8 +74,9 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
         for _ in range(repeats):
             _, ms = run_tiled(plan, probe.shape[1], n_src, frc_dof, frc_src, probe, hist_dof, tile)
             best = min(best, ms)
+        steps = max(int(plan.stats()["steps"]), 1)
         rec = dict(rows=r, rpw=w, remap=x, group=int(plan.info()["multi_group"]), tile=tile,
-                   ms_per_step=best / probe_steps)
+                   ms_per_step=best / steps)
         table.append(rec)
         return rec
 

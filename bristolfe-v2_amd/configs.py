@@ -244,17 +244,20 @@ def subdomain_block(n_dof_target, el_size=0.1e-3, rim=1e-3, n_src=64, rng=None):
 
 
 def fmc_multimodel(n_el=64, pitch=0.3e-3, n_sub=8, sub_dof=(1e5, 1e6), n_t=3000, seed=4,
-                   centre_freq=2e6, cycles=5, with_subdomains=True):
+                   centre_freq=2e6, cycles=5, with_subdomains=True, plate=None):
     """Full-matrix capture on the 20 mm x 10 mm plate (64 elements at 0.3 mm
     pitch, one source column per transmitting element, every element
     receiving) plus n_sub subdomain models of seeded uneven size, each with 64
     source columns on its inner boundary, all marched as ONE block-diagonal
     system with 64 columns. Returns the combined K, C, M and 0-based forcing
-    entries (dof, column) and monitored DOFs."""
+    entries (dof, column) and monitored DOFs. ``plate`` overrides the plate's
+    width and height (a dict, for reduced test models; the element size stays
+    0.1 mm, that of the subdomain models)."""
     import scipy.sparse as sp
 
     rng = np.random.default_rng(seed)
-    main = array_plate(n_el=n_el, pitch=pitch, n_t=n_t, centre_freq=centre_freq, cycles=cycles)
+    main = array_plate(n_el=n_el, pitch=pitch, n_t=n_t, centre_freq=centre_freq, cycles=cycles,
+                       **(plate or {}))
     blocks = [dict(K=main["K"], C=main["C"], M=main["M"], src=main["el_gi"] - 1, n=main["K"].shape[0])]
     if with_subdomains:
         sizes = rng.uniform(sub_dof[0], sub_dof[1], n_sub)

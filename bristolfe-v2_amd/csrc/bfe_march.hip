@@ -1243,6 +1243,7 @@ struct bfe_plan_s {
   int32_t ebe_kernel = 1;            // 1: pattern kernel when the mesh matches one
   int32_t use_graph = 0;             // single-source march captured in a hipGraph
   int32_t use_persistent = 0;        // single-source CSR march as one cooperative launch
+  int32_t persistent_grid = 0;       // cap on its workgroups (0: as many as fit at once)
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
@@ -1597,8 +1598,18 @@ int resolve_group(bfe_plan_s& H, int rpw) {
   return G;
 }
 
-// 0-based first column of F with a non-zero (or NaN) entry; -1 if none.
+// 0-based first step of the march, -1 if none: v6:163-166 takes
+// min(find(sum(abs(forcing_functions)))). With several rows the sum runs down
+// each column, so the march starts at the first column with a non-zero (or
+// NaN) entry. A single row (one signal broadcast to every forcing DOF) sums
+// to one scalar, so the march starts at step 0 whenever any sample is
+// non-zero, leading zero samples included.
 int64_t first_forced_step(const double* F, int32_t rows, int32_t n_t) {
+  if (rows == 1) {
+    for (int64_t t = 0; t < n_t; ++t)
+      if (F[t] != 0.0) return 0;  // NaN compares unequal to 0 as in MATLAB find()
+    return -1;
+  }
   for (int64_t t = 0; t < n_t; ++t) {
     double s = 0.0;
     for (int32_t r = 0; r < rows; ++r) s += std::fabs(F[t * rows + r]);
@@ -1738,7 +1749,8 @@ void launch_persistent(const bfe_plan_s& H, int64_t nblk, int64_t t0, int32_t n_
   BFE_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, H.device));
   const int64_t cap = static_cast<int64_t>(bpcu) * ncu;
   if (cap < 1) throw hip_failure("persistent march: no co-resident workgroups");
-  const int64_t grid = std::min(nblk, cap);
+  int64_t grid = std::min(nblk, cap);
+  if (H.persistent_grid > 0) grid = std::min<int64_t>(grid, H.persistent_grid);
   void* args[] = {&a};
   BFE_HIP(hipLaunchCooperativeKernel(fn, dim3(static_cast<uint32_t>(grid)), dim3(kBlock), args, 0,
                                      H.stream));
@@ -3144,6 +3156,10 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         break;
       case BFE_TUNE_PERSISTENT:
         h->use_persistent = value ? 1 : 0;
+        break;
+      case BFE_TUNE_PERSISTENT_GRID:
+        if (value < 0) throw std::invalid_argument("persistent grid cap must be >= 0");
+        h->persistent_grid = value;
         break;
       case BFE_TUNE_EBE_KERNEL:
         h->ebe_kernel = value ? 1 : 0;

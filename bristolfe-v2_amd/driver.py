@@ -29,13 +29,20 @@ from .solver import field_frames, fn_explicit_dynamic_solver_v6, make_plan
 
 
 def fn_nds_and_dfs_to_gi(nds, dfs, gl_lookup):
-    """(node, dof) -> 1-based global index; invalid pairs dropped
-    (fn_BristolFE_v2.m:159-168)."""
+    """(node, dof) -> 1-based global index (fn_BristolFE_v2.m:159-168). Pairs
+    whose gl_lookup entry is 0 (a DOF the node does not carry) are dropped,
+    as there; a node or DOF number outside gl_lookup raises IndexError, as
+    MATLAB's gl_lookup(nds(i), dfs(i)) does."""
     nds = np.asarray(nds, dtype=np.int64).ravel()
     dfs = np.asarray(dfs, dtype=np.int64).ravel()
-    gi = np.zeros(nds.size, dtype=np.int64)
-    ok = (nds >= 1) & (nds <= gl_lookup.shape[0]) & (dfs >= 1) & (dfs <= gl_lookup.shape[1])
-    gi[ok] = gl_lookup[nds[ok] - 1, dfs[ok] - 1]
+    if nds.size != dfs.size:
+        raise ValueError("nds and dfs must have the same number of entries")
+    bad = (nds < 1) | (nds > gl_lookup.shape[0]) | (dfs < 1) | (dfs > gl_lookup.shape[1])
+    if bad.any():
+        i = int(np.nonzero(bad)[0][0])
+        raise IndexError(f"(node {nds[i]}, dof {dfs[i]}) is outside gl_lookup "
+                         f"({gl_lookup.shape[0]} nodes x {gl_lookup.shape[1]} DOFs)")
+    gi = gl_lookup[nds - 1, dfs - 1].astype(np.int64)
     valid = gi > 0
     return gi[valid], nds[valid], dfs[valid], valid
 

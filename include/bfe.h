@@ -265,6 +265,11 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * marches on the CSR operator only (not with an element operator in use, not
  * bfe_run_checkpointed); no kernel sampling. Default 0. */
 #define BFE_TUNE_PERSISTENT 9
+/* at most this many workgroups in the persistent march's launch (0, the
+ * default: as many as the device holds at once, at most one per row block).
+ * With fewer workgroups than row blocks each workgroup sweeps blocks
+ * lb, lb + grid, ... between the grid barriers; results do not change. */
+#define BFE_TUNE_PERSISTENT_GRID 10
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3

@@ -23,6 +23,8 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from .v6 import ti_start_of
+
 LIB = Path(__file__).resolve().parent / "liboracle_march.so"
 _lib = None
 
@@ -99,12 +101,12 @@ class OracleMarch:
         F = np.atleast_2d(np.asarray(F, dtype=np.float64))
         fidx = np.ascontiguousarray(frc_idx0, np.int32).ravel()
         hidx = np.ascontiguousarray(hist_idx0, np.int32).ravel()
-        nz = np.nonzero(np.abs(F).sum(axis=0) != 0)[0] if fidx.size else np.zeros(0, np.int64)
+        ts = ti_start_of(F) if fidx.size else None
         q = hidx.size
         hist = np.zeros((q, n_t), order="F")
-        if nz.size == 0:
+        if ts is None:
             return hist, n_t + 1
-        t0 = int(nz[0])
+        t0 = ts - 1
         Fc = np.asfortranarray(F)
         K = self.K
         d = lambda a, t: a.ctypes.data_as(C.POINTER(t))

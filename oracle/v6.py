@@ -7,7 +7,9 @@ Line map:
   v6:112-120   field frames at ti = 1:N:n_t, zero-initialised
   v6:122-123   diag_M = spdiags(sum(M)), inv_M = spdiags(1 ./ sum(M))
   v6:130-139   A, B, D per solver mode; 'X \\ Y' for the current-step mode
-  v6:163-166   ti_start = first column with any non-zero forcing
+  v6:163-166   ti_start = min(find(sum(abs(F)))): with several rows the first
+               column holding a non-zero entry; a single row sums to one
+               scalar, so ti_start = 1 whenever any sample is non-zero
   v6:180-213   f(idx) = F(:, ti); u = A*f + B*u1 + D*u2; hist(:, ti) = u(h);
                frame (u - u1)/dt or u; shift
 """
@@ -73,7 +75,14 @@ def abd_matrices(K, C, M, dt, solver_mode):
 
 
 def ti_start_of(forcing_functions):
+    """v6:163-166, ``min(find(sum(abs(forcing_functions))))``, 1-based; None
+    when find is empty. MATLAB's ``sum`` runs down the first non-singleton
+    dimension: down each column of a p x n_t matrix (p > 1), but along the
+    row of a 1 x n_t signal, which gives one scalar and so ti_start = 1 as
+    soon as any sample is non-zero (NaN included)."""
     F = np.atleast_2d(np.asarray(forcing_functions, dtype=np.float64))
+    if F.shape[0] == 1:
+        return 1 if np.any(F != 0) else None
     nz = np.nonzero(np.abs(F).sum(axis=0) != 0)[0]
     return int(nz[0]) + 1 if nz.size else None
 

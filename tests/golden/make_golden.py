@@ -10,7 +10,11 @@ fully specified inputs:
 
   stability_mesh{0,1,2}  the tiny meshes of tests/stability.m:41-58 (one
                          steel element, one water element, the pair with an
-                         ASI2D2 interface element), K and C dense
+                         ASI2D2 interface element), K and C dense; one
+                         broadcast signal row whose first 9 samples are
+                         zero, and velocity frames every 4 steps, so the
+                         frames before the signal carry their times
+                         (v6:163-166 starts a single-row march at ti = 1)
   coupled_small          aluminium under water with an interface and an
                          absorbing layer (CSR), two forcing rows with
                          leading zeros, 'KE' field frames every 50 steps
@@ -52,11 +56,12 @@ def _save(name, K, C, m, dt, n_t, fi, F, hi, field_every=0):
     M = sp.diags(m)
     for k, mode in enumerate(MODES):
         every = field_every if field_every else np.inf
-        h, f, _, ti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, every, "KE", mode)
+        h, f, ft, ti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, every, "KE", mode)
         rec[f"hist_mode{k}"] = h
         rec[f"ti_start_mode{k}"] = np.int64(ti)
         if field_every:
             rec[f"field_mode{k}"] = f
+            rec[f"field_time_mode{k}"] = ft
     np.savez_compressed(OUT / f"{name}.npz", **rec)
     print(name, K.shape[0], "DOF,", K.nnz, "entries,", len(hi), "monitors x", n_t, "steps")
 
@@ -84,8 +89,9 @@ def stability_meshes():
         lam = np.max(np.abs(np.linalg.eigvals(np.diag(1 / m) @ K.toarray())))
         dt = 0.5 * 2 / np.sqrt(lam)
         n_t = 60
-        F = np.sin(np.arange(n_t) * 0.3) * np.exp(-0.05 * np.arange(n_t))
-        _save(f"stability_mesh{k}", K, C, m, dt, n_t, [n], F, np.arange(1, n + 1))
+        s = np.arange(n_t - 9)
+        F = np.concatenate([np.zeros(9), np.sin(s * 0.3 + 0.2) * np.exp(-0.05 * s)])
+        _save(f"stability_mesh{k}", K, C, m, dt, n_t, [n], F, np.arange(1, n + 1), field_every=4)
 
 
 def coupled_small():

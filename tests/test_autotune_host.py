@@ -34,10 +34,11 @@ class SyntheticPlan:
         shape = dict(self.shape, group=self.used_group(), columns=S)
         self.requests.append(shape)
         self._ms = self.cost(shape) * (n_t - 1)
+        self._steps = n_t - 1
         return np.zeros((len(hist_dof), S, n_t)), 1
 
     def stats(self):
-        return {"loop_ms": self._ms}
+        return {"loop_ms": self._ms, "steps": self._steps}
 
 
 def _cost(s):
@@ -77,3 +78,40 @@ def test_probe_starts_at_the_first_forced_sample():
     autotune.tune_batched(P(_cost), 8, fd, np.arange(8), F, fd, probe_steps=20, repeats=1,
                           rows=(1, 2), rpws=(1, 2), tiles=(64,))
     assert all(n == 37 + 20 for n, _ in seen)
+
+
+def test_run_tiled_passes_each_pass_its_own_signal_rows():
+    """One signal row per forcing entry (p rows) with a tile narrower than the
+    sources: every pass gets exactly the rows of its own entries, and the
+    assembled history puts each pass's columns in place."""
+    calls = []
+
+    class P:
+        def run_batched(self, n_t, S, frc_dof, frc_src, frcs, hist_dof):
+            frcs = np.atleast_2d(frcs)
+            assert frcs.shape[0] in (1, len(frc_dof))
+            calls.append((S, np.array(frc_dof), np.array(frc_src), frcs.copy()))
+            h = np.zeros((len(hist_dof), S, n_t))
+            for k, (d, s_) in enumerate(zip(frc_dof, frc_src)):
+                h[0, s_, :] += d * frcs[k if frcs.shape[0] > 1 else 0]
+            return h, 1
+
+        def stats(self):
+            return {"loop_ms": 1.0, "steps": 1}
+
+    n_src, n_t = 10, 6
+    fd = np.arange(20) + 100
+    fs = np.repeat(np.arange(n_src), 2)
+    F = np.arange(20 * n_t, dtype=float).reshape(20, n_t)
+    h, ms = autotune.run_tiled(P(), n_t, n_src, fd, fs, F, [0], 4)
+    assert [c[0] for c in calls] == [4, 4, 2] and ms == 3.0
+    for S, d, s_, rows in calls:
+        np.testing.assert_array_equal(rows, F[np.isin(fd, d)])
+    ref = np.zeros((1, n_src, n_t))
+    for k in range(20):
+        ref[0, fs[k]] += fd[k] * F[k]
+    np.testing.assert_array_equal(h, ref)
+    # one broadcast row goes to every pass unchanged
+    calls.clear()
+    autotune.run_tiled(P(), n_t, n_src, fd, fs, F[:1], [0], 4)
+    assert all(c[3].shape[0] == 1 for c in calls)

@@ -33,14 +33,28 @@ def _load(name):
 def test_oracle_reproduces_fixture(name, k):
     z, K, C, M, t, every = _load(name)
     assert "project oracle" in str(z["provenance"])
-    h, f, _, ti = ov6.explicit_dynamic_solver_v6(K, C, M, t, z["frc_gi"], z["frcs"], z["hist_gi"],
-                                                 every, "KE", MODES[k])
+    h, f, ft, ti = ov6.explicit_dynamic_solver_v6(K, C, M, t, z["frc_gi"], z["frcs"],
+                                                  z["hist_gi"], every, "KE", MODES[k])
     ref = z[f"hist_mode{k}"]
     assert np.linalg.norm(ref) > 0
     assert ti == int(z[f"ti_start_mode{k}"])
     assert rel_l2(h, ref) < 1e-13
     if np.isfinite(every):
         assert rel_l2(f, z[f"field_mode{k}"]) < 1e-13
+        np.testing.assert_array_equal(ft, z[f"field_time_mode{k}"])
+
+
+def test_single_row_fixtures_start_at_the_first_step():
+    """The stability fixtures force with one broadcast row whose first 9
+    samples are zero: v6:163-166 starts such a march at ti = 1, so the frames
+    at ti = 1, 5, 9 hold zero velocity and carry their own times."""
+    for name in CASES[:3]:
+        z, K, C, M, t, every = _load(name)
+        assert z["frcs"].shape[0] == 1 and not z["frcs"][0, :9].any() and z["frcs"][0, 9] != 0
+        assert int(z["ti_start_mode0"]) == 1
+        ft = z["field_time_mode0"]
+        np.testing.assert_array_equal(ft, t[::every])
+        assert not z["field_mode0"][:, :3].any() and z["field_mode0"][:, 3:].any()
 
 
 def test_fixtures_cover_interface_damping_and_tets():
@@ -58,8 +72,9 @@ def test_fixtures_cover_interface_damping_and_tets():
 def test_hip_march_matches_fixture(name, k):
     print("case: golden", name, MODES[k], flush=True)
     z, K, C, M, t, every = _load(name)
-    h, f, _, _ = PKG.fn_explicit_dynamic_solver_v6(K, C, M, t, z["frc_gi"], z["frcs"], [], [],
-                                                   z["hist_gi"], every, 1, "KE", MODES[k])
+    h, f, _, ft = PKG.fn_explicit_dynamic_solver_v6(K, C, M, t, z["frc_gi"], z["frcs"], [], [],
+                                                    z["hist_gi"], every, 1, "KE", MODES[k])
     assert rel_l2(h, z[f"hist_mode{k}"]) < 1e-10
     if np.isfinite(every):
         assert rel_l2(f, z[f"field_mode{k}"]) < 1e-10
+        np.testing.assert_array_equal(ft, z[f"field_time_mode{k}"])

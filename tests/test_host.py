@@ -103,9 +103,13 @@ def test_interface_elements_orientation_and_count():
 
 def test_driver_index_mapping_and_options_query():
     gl = np.array([[1, 2, 0, 0], [0, 0, 0, 3], [4, 5, 0, 6]])
-    gi, nds, dfs, valid = PKG.fn_nds_and_dfs_to_gi([1, 2, 3, 2, 9], [2, 1, 4, 4, 1], gl)
+    gi, nds, dfs, valid = PKG.fn_nds_and_dfs_to_gi([1, 2, 3, 2, 1], [2, 1, 4, 4, 3], gl)
     np.testing.assert_array_equal(gi, [2, 6, 3])
     np.testing.assert_array_equal(valid, [True, False, True, True, False])
+    # gl_lookup(nds(i), dfs(i)) with an index beyond gl_lookup errors in MATLAB
+    for nds_, dfs_ in (([1, 9], [1, 1]), ([1, 2], [1, 5]), ([0], [1]), ([1], [0])):
+        with pytest.raises(IndexError):
+            PKG.fn_nds_and_dfs_to_gi(nds_, dfs_, gl)
     opts = PKG.fn_FE_entry_point(None, None, None, {"max_damping": 1.0})
     assert opts["solver_mode"] == "vel at curent time step"
     assert opts["max_damping"] == 1.0 and opts["damping_power_law"] == 3

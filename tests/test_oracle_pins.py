@@ -167,6 +167,10 @@ def test_energy_conservation_solid():
 
 
 def test_ti_start_shift_and_linearity():
+    """v6:163-166: ti_start = min(find(sum(abs(F)))). MATLAB's sum of a 1 x n_t
+    row is one scalar, so a single (broadcast) signal starts the march at
+    ti = 1 even when its first samples are zero; with p > 1 rows the sum runs
+    down the columns and the march starts at the first non-zero column."""
     mod, matls = solid_plate(1e-3, 1e-3, 0.25e-3)
     K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2]})
     dt = 2e-9
@@ -177,12 +181,33 @@ def test_ti_start_shift_and_linearity():
     G = np.concatenate([np.zeros(17), F[:-17]])
     fi, hi = [3], [3, 7, 11]
     h1, _, _, t1 = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi)
-    h2, _, _, t2 = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, G, hi)
+    h2, f2, ft2, t2 = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, G, hi, 5)
     h3, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, 2 * F, hi)
-    assert t1 == 2 and t2 == 19   # F[0] = sin(0) = 0
+    assert t1 == 1 and t2 == 1    # one row: the scalar sum is non-zero
     assert np.all(h2[:, :18] == 0)
     np.testing.assert_array_equal(h2[:, 17:], h1[:, :n_t - 17])
     np.testing.assert_array_equal(h3, 2 * h1)
+    # every frame is written, those before the first non-zero sample with zero
+    # velocity and their own time
+    frames = np.arange(1, n_t + 1, 5)
+    np.testing.assert_array_equal(ft2, t[frames - 1])
+    assert not f2[:, frames <= 18].any() and f2[:, frames > 19].any()
+    # two rows (the second one all zero): the first column with a non-zero entry
+    G2 = np.vstack([G, np.zeros(n_t)])
+    h4, f4, ft4, t4 = ov6.explicit_dynamic_solver_v6(K, C, M, t, [3, 8], G2, hi, 5)
+    assert t4 == 19
+    np.testing.assert_array_equal(h4, h2)
+    np.testing.assert_array_equal(ft4, np.where(frames >= 19, t[frames - 1], 0.0))
+    np.testing.assert_array_equal(f4[:, frames >= 19], f2[:, frames >= 19])
+    # NaN counts as non-zero in find(), for one row and for several
+    Gn = G.copy()
+    Gn[30] = np.nan
+    assert ov6.ti_start_of(Gn) == 1
+    assert ov6.ti_start_of(np.vstack([Gn, np.zeros(n_t)])) == 19
+    Z = np.zeros((2, n_t))
+    Z[1, 30] = np.nan
+    assert ov6.ti_start_of(Z) == 31
+    assert ov6.ti_start_of(np.zeros(n_t)) is None
 
 
 def test_all_zero_forcing_gives_zero_outputs():
