@@ -40,7 +40,7 @@ EXPORTS = (
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
     "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
     "bfe_assemble_device_fill", "bfe_assemble_device_free", "bfe_convolve",
-    "bfe_disps_to_forces", "bfe_element_templates", "bfe_element_centres",
+    "bfe_disps_to_forces", "bfe_disps_to_forces_noalloc", "bfe_element_templates", "bfe_element_centres",
     "bfe_points_in_polygon", "bfe_dist_points_to_polygon",
 )
 
@@ -167,6 +167,8 @@ def load() -> C.CDLL:
                                         _f64p, _f64p, C.c_double, C.c_int32, _f64p, _i32p,
                                         C.c_int32, C.c_int32, C.c_int64, _i32p, _f64p]
     lib.bfe_disps_to_forces.restype = C.c_int
+    lib.bfe_disps_to_forces_noalloc.argtypes = lib.bfe_disps_to_forces.argtypes
+    lib.bfe_disps_to_forces_noalloc.restype = C.c_int
     lib.bfe_element_templates.argtypes = [C.c_int32, C.c_int64, C.c_int32, _f64p, C.c_int64,
                                           C.c_int32, _i32p, _i32p, _i64p, C.c_int32, _i32p, _i64p,
                                           _i32p, _f64p]
@@ -516,10 +518,14 @@ def convolve(f, g, g_per_signal, device=0):
     return h
 
 
-def disps_to_forces(K, Cm, m, dt, disp, lyrs, in_or_out, solver_mode, device=0):
+def disps_to_forces(K, Cm, m, dt, disp, lyrs, in_or_out, solver_mode, device=0,
+                    no_alloc=False):
     """bfe_disps_to_forces with scipy CSR K, C; disp (n, n_t). Returns
-    (force_idx 0-based, forces (n_force, n_t))."""
+    (force_idx 0-based, forces (n_force, n_t)). no_alloc=True calls
+    bfe_disps_to_forces_noalloc (no device allocation; at most 64 MiB of
+    device arrays)."""
     lib = load()
+    fn = lib.bfe_disps_to_forces_noalloc if no_alloc else lib.bfe_disps_to_forces
     n = K.shape[0]
     K = K.tocsr()
     Cm = Cm.tocsr()
@@ -535,12 +541,10 @@ def disps_to_forces(K, Cm, m, dt, disp, lyrs, in_or_out, solver_mode, device=0):
     nf = int(np.count_nonzero((lyrs == 2) | (lyrs == 3)))
     idx = np.empty(nf, np.int32)
     out = np.empty((nf, n_t), np.float64, order="F")
-    check(lib.bfe_disps_to_forces(int(device), n, ptr(kp, C.c_int64), ptr(kc, C.c_int32),
-                                  ptr(kv, C.c_double), ptr(cp, C.c_int64), ptr(cc, C.c_int32),
-                                  ptr(cv, C.c_double), ptr(m, C.c_double), float(dt), n_t,
-                                  disp.ctypes.data_as(_f64p), ptr(lyrs, C.c_int32),
-                                  int(in_or_out), int(solver_mode), nf, ptr(idx, C.c_int32),
-                                  out.ctypes.data_as(_f64p)))
+    check(fn(int(device), n, ptr(kp, C.c_int64), ptr(kc, C.c_int32), ptr(kv, C.c_double),
+             ptr(cp, C.c_int64), ptr(cc, C.c_int32), ptr(cv, C.c_double), ptr(m, C.c_double),
+             float(dt), n_t, disp.ctypes.data_as(_f64p), ptr(lyrs, C.c_int32), int(in_or_out),
+             int(solver_mode), nf, ptr(idx, C.c_int32), out.ctypes.data_as(_f64p)))
     return idx, out
 
 

@@ -454,6 +454,28 @@ bfe_status bfe_disps_to_forces(int32_t device, int64_t n, const int64_t* k_rowpt
                                double* force);
 
 /*
+ * bfe_disps_to_forces_noalloc — bfe_disps_to_forces that allocates nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). Arguments, results and errors are those of
+ * bfe_disps_to_forces, computed by the same kernel. Every device array of the
+ * call (the staged K, C, m, layers, force rows and displacements, and the
+ * forces) lives in one static 64 MiB device array per GPU, held from the
+ * library's load until the process ends; a call whose arrays need more
+ * (8 (4 n + 2 (nnz_K + nnz_C) + n n_t + n_force n_t) bytes, each sub-array
+ * rounded up to 256 bytes) returns BFE_ERR_ARG. Calls on one GPU share that
+ * array, so calls that overlap on one GPU (from several threads) race; calls
+ * on different GPUs do not.
+ */
+bfe_status bfe_disps_to_forces_noalloc(int32_t device, int64_t n, const int64_t* k_rowptr,
+                                       const int32_t* k_col, const double* k_val,
+                                       const int64_t* c_rowptr, const int32_t* c_col,
+                                       const double* c_val, const double* m, double dt,
+                                       int32_t n_t, const double* disp, const int32_t* lyrs,
+                                       int32_t in_or_out, int32_t solver_mode, int64_t n_force,
+                                       int32_t* force_idx, double* force);
+
+/*
  * Host mesh preprocessing (C++, multi-threaded), for models of millions of
  * elements. Element tables are taken as the model holds them: n_el x nper,
  * row-major, 1-based node numbers, 0 = absent. Points are n_pts x 2,
