@@ -224,14 +224,25 @@ def adjoint_plate(width=30e-3, height=15e-3, el_size=0.05e-3, n_el=16, pitch=1e-
                       f"{n_el} tx/rx, {n_t} steps"))
 
 
-def subdomain_block(n_dof_target, el_size=0.1e-3, rim=1e-3, n_src=64, rng=None):
-    """Steel rectangle (2:1) of about n_dof_target DOF (DOF 1, 2) with an
-    absorbing rim on every side, and n_src forcing/monitoring nodes spread
-    along the inner top edge."""
+def _whole(k, el_size):
+    """The largest length of k elements of el_size from which the mesher's
+    ceil(length / el_size) gives k (k * el_size itself can round above k)."""
+    L = k * el_size
+    while np.ceil(L / el_size) > k:
+        L = np.nextafter(L, 0.0)
+    return L
+
+
+def subdomain_block(n_dof_target, el_size=0.1e-3, rim=1e-3, n_src=64, rng=None, height=20e-3):
+    """Steel rectangle of about n_dof_target DOF (DOF 1, 2), `height` tall
+    and as wide as the DOF count asks, both whole multiples of el_size (so
+    every block has the elements, the node-column height and the stable time
+    step of the others), with an absorbing rim on every side, and n_src
+    forcing/monitoring nodes spread along the inner top edge."""
     rng = np.random.default_rng(rng)
-    nodes = n_dof_target / 2.0
-    H = np.sqrt(nodes / 2.0) * el_size
-    W = 2.0 * H
+    rows = int(round(height / el_size))
+    cols = max(int(round(n_dof_target / 2.0 / (rows + 1))) - 1, int(np.ceil(2.5 * rim / el_size)))
+    H, W = _whole(rows, el_size), _whole(cols, el_size)
     matls = [steel()]
     mod = _typ(_m.fn_rectangular_structured_mesh(_rect(0, 0, W, H), el_size), matls)
     inner = _rect(rim, rim, W - rim, H - rim)
@@ -240,11 +251,29 @@ def subdomain_block(n_dof_target, el_size=0.1e-3, rim=1e-3, n_src=64, rng=None):
     K, C, M, gl = _assemble(mod, matls, opts)
     xs = np.sort(rim + (W - 2 * rim) * rng.random(n_src))
     nd = _m.find_nodes_at_points(mod["nds"], np.stack([xs, np.full(n_src, H - rim)], axis=1))
-    return dict(K=K, C=C, M=M, gl=gl, src=gl[nd - 1, 1] - 1, n=K.shape[0])
+    return dict(K=K, C=C, M=M, gl=gl, src=gl[nd - 1, 1] - 1, n=K.shape[0], mod=mod)
+
+
+def union_mesh(mods):
+    """One model holding several disjoint meshes, node and element numbers
+    offset in order. With every node carrying the same DOFs its node-major
+    DOF numbering is the meshes' numberings one after the other, so its
+    assembled K, C, M are the block-diagonal matrices of the meshes."""
+    offs = np.cumsum([0] + [m["nds"].shape[0] for m in mods])
+    els = []
+    for m, o in zip(mods, offs[:-1]):
+        e = np.asarray(m["els"], dtype=np.int64).copy()
+        e[e > 0] += o
+        els.append(e)
+    return {"nds": np.vstack([m["nds"] for m in mods]), "els": np.vstack(els),
+            "el_mat_i": np.concatenate([np.asarray(m["el_mat_i"]) for m in mods]),
+            "el_abs_i": np.concatenate([np.asarray(m.get("el_abs_i", np.zeros(len(m["el_mat_i"]))),
+                                                   dtype=np.float64) for m in mods]),
+            "el_typ_i": np.concatenate([np.asarray(m["el_typ_i"], dtype=object) for m in mods])}
 
 
 def fmc_multimodel(n_el=64, pitch=0.3e-3, n_sub=8, sub_dof=(1e5, 1e6), n_t=3000, seed=4,
-                   centre_freq=2e6, cycles=5, with_subdomains=True, plate=None):
+                   centre_freq=2e6, cycles=5, with_subdomains=True, plate=None, sub_height=20e-3):
     """Full-matrix capture on the 20 mm x 10 mm plate (64 elements at 0.3 mm
     pitch, one source column per transmitting element, every element
     receiving) plus n_sub subdomain models of seeded uneven size, each with 64
@@ -252,17 +281,19 @@ def fmc_multimodel(n_el=64, pitch=0.3e-3, n_sub=8, sub_dof=(1e5, 1e6), n_t=3000,
     system with 64 columns. Returns the combined K, C, M and 0-based forcing
     entries (dof, column) and monitored DOFs. ``plate`` overrides the plate's
     width and height (a dict, for reduced test models; the element size stays
-    0.1 mm, that of the subdomain models)."""
+    0.1 mm, that of the subdomain models); the subdomain models are
+    ``sub_height`` tall and as wide as their seeded DOF counts ask."""
     import scipy.sparse as sp
 
     rng = np.random.default_rng(seed)
     main = array_plate(n_el=n_el, pitch=pitch, n_t=n_t, centre_freq=centre_freq, cycles=cycles,
                        **(plate or {}))
-    blocks = [dict(K=main["K"], C=main["C"], M=main["M"], src=main["el_gi"] - 1, n=main["K"].shape[0])]
+    blocks = [dict(K=main["K"], C=main["C"], M=main["M"], src=main["el_gi"] - 1, n=main["K"].shape[0],
+                   mod=main["mod"])]
     if with_subdomains:
         sizes = rng.uniform(sub_dof[0], sub_dof[1], n_sub)
         for k in range(n_sub):
-            blocks.append(subdomain_block(sizes[k], n_src=n_el, rng=rng))
+            blocks.append(subdomain_block(sizes[k], n_src=n_el, rng=rng, height=sub_height))
     offs = np.cumsum([0] + [b["n"] for b in blocks])
     K = sp.block_diag([b["K"] for b in blocks], format="csr")
     C = sp.block_diag([b["C"] for b in blocks], format="csr")
@@ -276,7 +307,15 @@ def fmc_multimodel(n_el=64, pitch=0.3e-3, n_sub=8, sub_dof=(1e5, 1e6), n_t=3000,
         fsrc.append(np.arange(n_el))
         mon.append(b["src"] + o)
     t = np.arange(n_t) * dt
+    # the same system as one model of disjoint meshes (for the element operator)
+    mod = union_mesh([b["mod"] for b in blocks])
+    n_nodes = mod["nds"].shape[0]
+    gl = np.zeros((n_nodes, 2), dtype=np.int64)
+    gl[:, 0] = 2 * np.arange(n_nodes) + 1
+    gl[:, 1] = 2 * np.arange(n_nodes) + 2
+    assert 2 * n_nodes == K.shape[0]
     return dict(K=K, C=C, M=M, dt=dt, time=t, frcs=_m.hann_toneburst(t, centre_freq, cycles)[None, :],
+                mod=mod, matls=[steel()], fe_options={"dof_to_use": [1, 2]}, gl_lookup=gl,
                 frc_dof=np.concatenate(fdof), frc_src=np.concatenate(fsrc), hist_dof=np.concatenate(mon),
                 n_src=n_el, model_sizes=[int(b["n"]) for b in blocks], offsets=offs,
                 main_el_dof=main["el_gi"] - 1,

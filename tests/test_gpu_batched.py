@@ -61,8 +61,8 @@ def test_multimodel_block_diagonal_launch():
     single-source run of the same forcing entries to 1e-10."""
     print("case: multi-model block-diagonal launch", flush=True)
     from bristolfe_v2_amd import configs
-    c = configs.fmc_multimodel(pitch=0.1e-3, n_sub=3, sub_dof=(2e3, 8e3), n_t=250,
-                               plate=dict(width=8e-3, height=4e-3))
+    c = configs.fmc_multimodel(pitch=0.1e-3, n_sub=3, sub_dof=(4e3, 12e3), n_t=250,
+                               plate=dict(width=8e-3, height=4e-3), sub_height=4e-3)
     K, C, M, t = c["K"], c["C"], c["M"], c["time"]
     sizes = c["model_sizes"]
     assert len(sizes) == 4 and len(set(sizes)) == 4

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
-from conftest import assemble, coupled_block, rel_l2, solid_plate, steel, water
+from conftest import PKG, assemble, coupled_block, rel_l2, solid_plate, steel, water
 from oracle import assembly as oa
 from oracle import elements as oe
 from oracle import march as om
@@ -402,3 +402,34 @@ def test_amplification_matrix_spectral_radius_matches_dt_limit():
         # rigid-body modes give a defective eigenvalue 1 (Jordan block): 1 + O(sqrt(eps))
         assert op.max_amplification(M, C, K, 0.95 * dt_crit, mode) <= 1 + 1e-6
         assert op.max_amplification(M, C, K, 1.05 * dt_crit, mode) > 1 + 1e-3
+
+
+@pytest.mark.parametrize("mode", ["vel at curent time step", "vel at last half time step"])
+def test_abd_diagonal_x_equals_generic_formation(mode):
+    """The row-block formation of A, B, D for a diagonal C (the CPU baseline's
+    literal march on large models) gives the generic sparse formation's
+    matrices (v6:130-139) to rounding, on an absorbing plate, and the same
+    histories through the literal loop; t_end stops the loop after that
+    step."""
+    mod, matls = solid_plate(2e-3, 1e-3, 0.1e-3)
+    inner = np.array([[0.3e-3, 0.3e-3], [1.7e-3, 0.3e-3], [1.7e-3, 5e-3], [0.3e-3, 5e-3]])
+    mod = PKG.mesh.fn_add_absorbing_layer(mod, inner, 0.3e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2], "damping_power_law": 2})
+    assert C.diagonal().max() > 0
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    ref = ov6.abd_matrices(K, C, M, dt, mode)
+    got = ov6.abd_diagonal_x(K, C, M, dt, mode, chunk=97)
+    for a, b in zip(got, ref):
+        assert abs(a - b).max() <= 1e-15 * abs(b).max()
+    t = np.arange(200) * dt
+    F = PKG.mesh.hann_toneburst(t, 5e6, 3)
+    fi, hi = [gl[50, 1]], np.arange(1, K.shape[0] + 1, 7)
+    h1, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, solver_mode=mode, abd=ref)
+    h2, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, solver_mode=mode, abd=got,
+                                                 t_end=120)
+    assert np.linalg.norm(h1[:, :120] - h2[:, :120]) <= 1e-12 * np.linalg.norm(h1[:, :120])
+    assert not h2[:, 120:].any() and h1[:, 120:].any()
+    Cx = C.tolil()
+    Cx[0, 1] = 1.0
+    with pytest.raises(ValueError):
+        ov6.abd_diagonal_x(K, Cx.tocsr(), M, dt, mode)
