@@ -392,10 +392,7 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
         plan = make_plan(K, C, M, c["dt"], "vel at curent time step", prec, element_op=op)
         if shape is not None or prec == "single":
             sh = shape if shape is not None else res["double"]["best"]
-            for key, v in ((_lib.TUNE_MULTI_ROWS_PER_BLOCK, sh["rows"]),
-                           (_lib.TUNE_MULTI_ROWS_PER_WAVE, sh["rpw"]),
-                           (_lib.TUNE_XCD_REMAP, sh["remap"]), (_lib.TUNE_MULTI_GROUP, sh["group"])):
-                plan.set_tuning(key, v)
+            autotune.apply(plan, sh)
             best, table = dict(sh), []
         elif autotune_full:
             best, table = autotune.tune_batched(plan, S, fd, fs, F, hd)
@@ -414,6 +411,22 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
         res[prec] = dict(plan=plan, best=best, table=table, wall=float(np.mean(walls)),
                          loop_ms=float(np.mean(loops)), hist=hb, steps=plan.stats()["steps"])
         log(f"{prec}: batched {S} columns: loop {np.mean(loops):.1f} ms, wall {np.mean(walls):.2f} s")
+    # both multi-source operators at the chosen shape, one full march each
+    plan = res["double"]["plan"]
+    ops = {}
+    for name, sel in (("csr", 0), ("element_product", 1)):
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, sel)
+        if sel:
+            plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, res["double"]["best"].get("npw") or 4)
+        hb, ms = autotune.run_tiled(plan, n_t, S, fd, fs, F, hd, res["double"]["best"]["tile"])
+        st = plan.stats()
+        if plan.info()["multi_operator"] != sel:
+            continue
+        ops[name] = {"loop_ms": round(ms, 1), "ms_per_step": round(ms / max(st["steps"], 1), 4),
+                     "rel_l2_vs_chosen": float(np.linalg.norm(hb - res["double"]["hist"]) /
+                                               np.linalg.norm(res["double"]["hist"]))}
+        log(f"operator {name}: {ops[name]}")
+    autotune.apply(plan, res["double"]["best"])
     # 64 plain single-source launches (FP64) with the single-source march's
     # operator choice ("auto": element-by-element on this mesh), forcing upload
     # and history download included
@@ -444,7 +457,12 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
     info = plan.info()
     speedup = plain_wall / res["double"]["wall"]
     steps = res["double"]["steps"]
-    bpd = (12.0 * info["nnz_k"] / n + 4.0) / S + 24.0 + (16.0 / S if info["damped"] else 0.0)
+    if res["double"]["best"].get("operator") == "ebe":
+        # per DOF-step-source: the element data and coefficients once per step
+        # for all S sources, u_n / u_{n-1} read and u_{n+1} written per source
+        bpd = (ebe.ebe_bytes_per_dof(op, n, info["damped"]) - 24.0) / S + 24.0
+    else:
+        bpd = (12.0 * info["nnz_k"] / n + 4.0) / S + 24.0 + (16.0 / S if info["damped"] else 0.0)
     gb = bpd * n * S * steps / (res["double"]["loop_ms"] / 1e3) / 1e9
     traffic, unc = read_traffic("fmc_batched", "k_step_multi")
     out = {
@@ -457,7 +475,10 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
         "roofline": {"bound": "hbm", "achieved": round(gb, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gb / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_uncertainty": unc,
-                     "kernel": (f"k_step_multi<double> ({S} sources, {res['double']['best']['rpw']} "
+                     "kernel": (f"k_step_ebe_multi<double,true,PatTriRect,2,true> ({S} sources, "
+                                f"{res['double']['best']['npw']} nodes per wavefront)"
+                                if res["double"]["best"].get("operator") == "ebe" else
+                                f"k_step_multi<double> ({S} sources, {res['double']['best']['rpw']} "
                                 f"rows per wavefront, {res['double']['best']['group']} rows per "
                                 f"gather)"),
                      "bytes_per_dof_step_source": round(bpd, 2)},
@@ -471,7 +492,9 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
                  "speedup_vs_fp64_batched": round(res["double"]["wall"] / res["single"]["wall"], 3),
                  "launch_shape": res["single"]["best"]},
         "autotune": {"chosen": res["double"]["best"], "candidates": len(res["double"]["table"]),
-                     "search": "full" if autotune_full else "reduced (shape and group stages)"},
+                     "search": ("full" if autotune_full else
+                                "reduced (shape, group and element-product stages)")},
+        "multi_source_operators": ops,
         "setup_s": {"model_build": round(build_s, 2), "element_operator": round(t_op, 2)},
         "subdomain_ops": subdomain_ops(c, h64, K, C, M),
     }

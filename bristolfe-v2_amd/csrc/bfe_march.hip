@@ -762,19 +762,39 @@ struct PatKuhn24 {  // cubic cells split into 6 tetrahedra around a main diagona
       {0x999, 0x05a, 0x264}, {0x5a9, 0x252, 0x06c}, {0x099, 0x95a, 0x4a4},
       {0x489, 0x372, 0x0b4}, {0x129, 0x912, 0x4ec}, {0x909, 0x132, 0x2f4}};
   static constexpr bool nzbit(int k, int c, int col) { return (nz[k][c] >> col) & 1u; }
+  // nxt[j]: the class whose node offset is class j's plus one (the next
+  // node along the fastest mesh index), -1 if none; checked against the
+  // group's offsets before a launch relies on it (k_step_ebe_multi)
+  static constexpr int8_t nxt[NC] = {1, -1, 3, -1, 5, -1, 7, 8, -1, 10, -1, 12, -1, 14, -1};
 };
 struct PatTriRect {  // squares split into two right triangles
   static constexpr int ID = 2, NPE = 3, NS = 6, NC = 7, SELF = 3;
   static constexpr int8_t cls[NS][NPE] = {{0, 1, 3}, {0, 2, 3}, {1, 3, 4},
                                           {2, 3, 5}, {3, 4, 6}, {3, 5, 6}};
   static constexpr bool nzbit(int, int, int) { return true; }
+  static constexpr int8_t nxt[NC] = {1, -1, 3, 4, -1, 6, -1};
 };
 struct PatTriIso {  // near-equilateral triangles
   static constexpr int ID = 3, NPE = 3, NS = 6, NC = 7, SELF = 2;
   static constexpr int8_t cls[NS][NPE] = {{0, 1, 2}, {0, 2, 3}, {1, 2, 4},
                                           {2, 3, 6}, {2, 4, 5}, {2, 5, 6}};
   static constexpr bool nzbit(int, int, int) { return true; }
+  static constexpr int8_t nxt[NC] = {-1, -1, -1, -1, -1, -1, -1};  // no sliding window
 };
+
+// Whether a group's class offsets (in DOFs) follow a pattern's nxt table:
+// off[nxt[j]] = off[j] + dpn wherever nxt[j] >= 0.
+template <typename PT>
+bool offsets_follow_next(const std::vector<int64_t>& doff, int dpn) {
+  if (static_cast<int>(doff.size()) != PT::NC) return false;
+  bool any = false;
+  for (int j = 0; j < PT::NC; ++j)
+    if (PT::nxt[j] >= 0) {
+      if (doff[PT::nxt[j]] != doff[j] + dpn) return false;
+      any = true;
+    }
+  return any;
+}
 
 // Class offsets (w - v) of a pattern's neighbours on regular nodes.
 template <int NC>
@@ -948,6 +968,190 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
         v, lb, n_nodes, n, r0, pm, sm, co, an, nbr, adj, el_scale, anchor, srows, stencil, alpha,
         beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, fld_prev, fld_is_vel, dt, I,
         coef_per_node);
+}
+
+// ---------------------------------------------------------------------------
+// Multi-source element-by-element step (the element product of the batched
+// march): u is n x S (source axis contiguous); one wavefront per node, lane =
+// source, so everything about the node (its masks, first DOF, element
+// scales, template rows) is wave-uniform and read through the scalar cache,
+// and each neighbour row of u1 is one coalesced S-value gather. A wavefront
+// marches npw consecutive nodes; along a run of regular nodes whose first
+// DOFs advance by DPN, the NC neighbour rows held in registers slide by one
+// node (x[j] <- x[nxt[j]]) and only the classes without a successor are
+// gathered: 3 of 7 node rows for triangles, 7 of 15 for the tets. The update,
+// forcing (f_src = lane), interface and monitor handling are those of
+// k_step_multi on the node's rows; the lists hold the group's local rows
+// v*DPN + c and are grouped per workgroup of this launch.
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false>
+__global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
+    int64_t n_nodes, int64_t nblk, int32_t npw, int32_t S, int32_t slide, int64_t n,
+    const int32_t* __restrict__ dof0, ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an,
+    const uint32_t* __restrict__ pmask, const uint32_t* __restrict__ smask,
+    const int32_t* __restrict__ nbr, const int32_t* __restrict__ adj,
+    const T* __restrict__ el_scale, const T* __restrict__ anchor, const T* __restrict__ srows,
+    const T* __restrict__ stencil, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, IfaceRows<T> I, int32_t coef_per_node) {
+  constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
+  constexpr uint32_t kAll = (1u << NS) - 1u;
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  const int64_t wave =
+      lb * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const bool active = lane < S;
+  const int64_t vbeg = wave * npw;
+  if (vbeg >= n_nodes) return;  // uniform across the wavefront
+  const int64_t vend = min(vbeg + npw, n_nodes);
+  auto load = [&](int64_t r) -> T { return active ? u1[r * S + lane] : T(0); };
+  // first DOF of neighbour j on a regular node (an offset out of range
+  // belongs to an absent neighbour, used only by empty slots)
+  auto reg = [&](int j, int64_t r0) -> int64_t {
+    const int64_t wr = r0 + co.off[j];
+    return (wr < 0 || wr > n - DPN) ? r0 : wr;
+  };
+  T x[NC][DPN];
+  int64_t rprev = -1;
+  bool win = false;
+  for (int64_t v = vbeg; v < vend; ++v) {
+    const uint32_t pm = pmask[v];
+    const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
+    const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
+    if (pm >> 31) {  // irregular node: neighbours from the class-major table
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        int64_t w = r0;
+        if (j != PT::SELF) {
+          const int32_t wj = nbr[static_cast<int64_t>(j) * n_nodes + v];
+          w = wj < 0 ? r0 : wj;
+        }
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) x[j][d] = load(w + d);
+      }
+      win = false;
+    } else if (win && slide && r0 == rprev + DPN) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {  // ascending: nxt[j] > j still holds the old row
+        // (j is a constant once unrolled: the branch and the index fold)
+        constexpr int8_t const* nx = PT::nxt;
+        if (nx[j] >= 0) {
+#pragma unroll
+          for (int d = 0; d < DPN; ++d) x[j][d] = x[nx[j] >= 0 ? nx[j] : j][d];
+        } else {
+          const int64_t w = reg(j, r0);
+#pragma unroll
+          for (int d = 0; d < DPN; ++d) x[j][d] = load(w + d);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        const int64_t w = j == PT::SELF ? r0 : reg(j, r0);
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) x[j][d] = load(w + d);
+      }
+      win = true;
+    }
+    rprev = r0;
+    T acc[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+    if (!ANCHOR && (pm & kAll) == kAll && sm == 0u) {
+      // every element present with scale 1: the fixed stencil sum_j B_j u_j
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+#pragma unroll
+        for (int d = 0; d < DPN; ++d)
+#pragma unroll
+          for (int c = 0; c < DPN; ++c) acc[c] += stencil[(j * DPN + c) * DPN + d] * x[j][d];
+    } else {
+      const bool irr = (pm >> 31) != 0u;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        if (((pm >> k) & 1u) == 0u) continue;  // empty slot (uniform branch)
+        T s = T(1);
+        if ((sm >> k) & 1u) {
+          if (ANCHOR && !irr) {
+            const int64_t a = v + an.base[k];
+            s = (a < 0 || a >= an.len) ? T(1) : anchor[a];
+          } else {
+            s = el_scale[adj[static_cast<int64_t>(k) * n_nodes + v]];
+          }
+        }
+        const T* R = srows + k * (DPN * KE);
+        T y[DPN];
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) y[c] = T(0);
+#pragma unroll
+        for (int b = 0; b < NPE; ++b)
+#pragma unroll
+          for (int d = 0; d < DPN; ++d)
+#pragma unroll
+            for (int c = 0; c < DPN; ++c)
+              if (!SP || PT::nzbit(k, c, b * DPN + d))
+                y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) acc[c] += s * y[c];
+      }
+    }
+    if (!active) continue;
+    // the update of the node's rows (k_step_multi's, per lane)
+    const uint32_t undamped = (pm >> 30) & 1u;
+    const int64_t l0 = v * DPN;
+    T a[DPN], b[DPN], un[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) {
+      const int64_t r = r0 + c;
+      const int64_t ci = coef_per_node ? v : r;
+      a[c] = x[PT::SELF][c];
+      b[c] = stream_load<true>(u2 + r * S + lane);
+      const T ka = alpha[ci] * acc[c];
+      if constexpr (DAMPED) {
+        const T be = undamped ? T(2) : beta[ci];
+        un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
+      } else {
+        un[c] = (T(2) * a[c] - b[c]) - ka;
+      }
+    }
+    for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i) {
+      const int64_t rr = L.f_row[i];
+      if (rr / DPN == v && L.f_src[i] == lane) {
+        const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+#pragma unroll
+        for (int c = 0; c < DPN; ++c)
+          if (rr == l0 + c) un[c] += f;
+      }
+    }
+    for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
+      const int64_t rr = L.i_row[i];
+      if (rr / DPN != v) continue;
+      const int32_t k = L.i_k[i];
+      T c1 = T(0), c2 = T(0);
+      for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
+        const int64_t cc = static_cast<int64_t>(I.ccol[p]) * S + lane;
+        c1 += I.cval[p] * u1[cc];
+        c2 += I.cval[p] * u2[cc];
+      }
+#pragma unroll
+      for (int c = 0; c < DPN; ++c)
+        if (rr == l0 + c) {
+          un[c] += I.gfix[k] * b[c] + I.sq * c1 + I.sr * c2;
+          if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) stream_store<true>(un[c], u + (r0 + c) * S + lane);
+    if (hist_prev != nullptr)
+      for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i) {
+        const int64_t rr = L.m_row[i];
+        if (rr / DPN == v) {
+#pragma unroll
+          for (int c = 0; c < DPN; ++c)
+            if (rr == l0 + c) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a[c];
+        }
+      }
+  }
 }
 
 // Pattern id of a layout's class table, 0 if none matches.
@@ -1244,6 +1448,10 @@ struct bfe_plan_s {
   int32_t use_graph = 0;             // single-source march captured in a hipGraph
   int32_t use_persistent = 0;        // single-source CSR march as one cooperative launch
   int32_t persistent_grid = 0;       // cap on its workgroups (0: as many as fit at once)
+  int32_t multi_ebe = 0;             // batched march on the element operator when attached
+  int32_t multi_npw = 4;             // nodes per wavefront of k_step_ebe_multi
+  int32_t multi_pipeline = 1;        // k_step_ebe_multi: neighbour rows slide along node runs
+  int32_t last_multi_op = BFE_OPERATOR_CSR;  // operator of the last batched run
   ~bfe_plan_s() {
     if (stream) {
       (void)hipSetDevice(device);
@@ -2498,6 +2706,163 @@ void launch_multi(const bfe_plan_s& H, int64_t nblk, int32_t rpb, int32_t rpw, i
     launch_multi_d<T, false>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, F_rows, t, q, hp, IR);
 }
 
+// Per-group lists of one batched run on the element operator (group-local
+// rows, grouped per workgroup of k_step_ebe_multi: 4 wavefronts of npw nodes).
+struct EbeMultiLists {
+  DevBuf fptr, frow, fsrc, ffrow, falpha, mptr, mrow, mk, iptr, irow, ik;
+  BlockLists lists() const {
+    BlockLists L;
+    L.fptr = fptr.as<int32_t>(); L.f_row = frow.as<int32_t>(); L.f_src = fsrc.as<int32_t>();
+    L.f_frow = ffrow.as<int32_t>();
+    L.mptr = mptr.as<int32_t>(); L.m_row = mrow.as<int32_t>(); L.m_k = mk.as<int32_t>();
+    L.iptr = iptr.as<int32_t>(); L.i_row = irow.as<int32_t>(); L.i_k = ik.as<int32_t>();
+    return L;
+  }
+};
+
+constexpr int kMultiWaves = kBlock / 64;  // wavefronts per workgroup of k_step_ebe_multi
+
+inline int64_t ebe_multi_blocks(const EbeGroup& Gr, int32_t npw) {
+  const int64_t per = static_cast<int64_t>(kMultiWaves) * npw;
+  return (Gr.n_nodes + per - 1) / per;
+}
+
+// Whether the batched march can run on the element operator: every group
+// has a pattern kernel.
+bool ebe_multi_ok(const bfe_plan_s& H) {
+  if (!H.ebe || !H.ebe_kernel) return false;
+  for (const auto& g : H.ebe->groups)
+    if (g.pattern == 0 || !has_pattern_kernel(g.pattern, g.dpn)) return false;
+  return true;
+}
+
+// Forcing (for one (dof, source) the last entry wins, as make_run_buffers),
+// monitor and interface lists of every group for one batched run.
+template <typename T>
+std::vector<EbeMultiLists> ebe_multi_lists(const bfe_plan_s& H, const EbeOp& E, int32_t npw,
+                                           const int32_t* frc_idx, const int32_t* frc_src,
+                                           int32_t p, int32_t frc_rows, const int32_t* hist_idx,
+                                           int32_t q) {
+  std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> key;
+  for (int32_t i = 0; i < p; ++i) key.push_back({{frc_idx[i], frc_src[i]}, i});
+  std::stable_sort(key.begin(), key.end(), [](auto& a, auto& b) { return a.first < b.first; });
+  const size_t ng = E.groups.size();
+  std::vector<std::vector<int32_t>> frows(ng), fsrc(ng), ffrow(ng), fidx(ng), mrows(ng), mks(ng),
+      irows(ng), iks(ng);
+  std::vector<std::vector<double>> fal(ng);
+  for (size_t i = 0; i < key.size(); ++i)
+    if (i + 1 == key.size() || key[i + 1].first != key[i].first) {
+      const int32_t r = key[i].first.first;
+      const int g = E.row_group[r];
+      if (g < 0) throw std::invalid_argument("forcing on a DOF that no element group owns");
+      frows[g].push_back(E.row_local[r]);
+      fsrc[g].push_back(key[i].first.second);
+      ffrow[g].push_back(frc_rows == 1 ? 0 : key[i].second);
+      fal[g].push_back(H.alpha[r]);
+    }
+  for (int32_t k = 0; k < q; ++k) {
+    const int32_t r = hist_idx[k];
+    const int g = E.row_group[r];
+    if (g < 0) throw std::invalid_argument("monitor on a DOF that no element group owns");
+    mrows[g].push_back(E.row_local[r]);
+    mks[g].push_back(k);
+  }
+  for (size_t k = 0; k < H.iface_host.size(); ++k) {
+    const int32_t r = H.iface_host[k];
+    const int g = E.row_group[r];
+    if (g < 0) throw std::invalid_argument("interface row that no element group owns");
+    irows[g].push_back(E.row_local[r]);
+    iks[g].push_back(static_cast<int32_t>(k));
+  }
+  std::vector<EbeMultiLists> out(ng);
+  for (size_t g = 0; g < ng; ++g) {
+    const EbeGroup& Gr = E.groups[g];
+    const int64_t nb = ebe_multi_blocks(Gr, npw);
+    const int64_t rpb = static_cast<int64_t>(kMultiWaves) * npw * Gr.dpn;
+    std::vector<int32_t> idx(frows[g].size());
+    for (size_t i = 0; i < idx.size(); ++i) idx[i] = static_cast<int32_t>(i);
+    Grouped gf = group_by_block(nb, rpb, frows[g], &fsrc[g], &ffrow[g]);
+    Grouped gfi = group_by_block(nb, rpb, frows[g], &idx, nullptr);
+    std::vector<double> fa(gfi.a.size());
+    for (size_t i = 0; i < fa.size(); ++i) fa[i] = fal[g][gfi.a[i]];
+    Grouped gm = group_by_block(nb, rpb, mrows[g], &mks[g], nullptr);
+    Grouped gi = group_by_block(nb, rpb, irows[g], &iks[g], nullptr);
+    EbeMultiLists& R = out[g];
+    R.fptr = upload(gf.ptr); R.frow = upload(gf.row); R.fsrc = upload(gf.a);
+    R.ffrow = upload(gf.b); R.falpha = upload(cast_vec<T>(fa));
+    R.mptr = upload(gm.ptr); R.mrow = upload(gm.row); R.mk = upload(gm.a);
+    R.iptr = upload(gi.ptr); R.irow = upload(gi.row); R.ik = upload(gi.a);
+  }
+  return out;
+}
+
+template <typename T, bool D, typename PT, int DPN>
+void launch_ebe_multi_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr,
+                          const EbeMultiLists& ML, int32_t npw, int32_t S, const T* u1,
+                          const T* u2, T* u, const T* F, int32_t F_rows, int32_t t, int32_t q,
+                          T* hp, const IfaceRows<T>& IR) {
+  ClassOffsets<PT::NC> co;
+  for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(Gr.doff[j]);
+  AnchorSlots<PT::NS> an{};
+  const bool anchored = PT::NS <= kAnchorMaxSlots && Gr.anchor.bytes > 0;
+  if (anchored) {
+    for (int k = 0; k < PT::NS; ++k)
+      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
+    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+  }
+  const int32_t slide = (H.multi_pipeline && offsets_follow_next<PT>(Gr.doff, DPN)) ? 1 : 0;
+  const int64_t nblk = ebe_multi_blocks(Gr, npw);
+  const BlockLists L = ML.lists();
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(kBlock), 0, H.stream, Gr.n_nodes, nblk, npw, S,
+                       slide, H.n, Gr.uniform ? nullptr : Gr.dof0.as<int32_t>(), co, an,
+                       Gr.pmask.as<uint32_t>(), Gr.smask.as<uint32_t>(), Gr.nbr.as<int32_t>(),
+                       Gr.adj.as<int32_t>(), Gr.scale.as<T>(),
+                       anchored ? Gr.anchor.as<T>() : static_cast<const T*>(nullptr),
+                       Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha_of<T>(Gr),
+                       D ? E.beta_of<T>(Gr) : nullptr, u1, u2, u, L, ML.falpha.as<T>(), F,
+                       F_rows, t, q, hp, IR, Gr.coef_node);
+  };
+  if constexpr (PT::NS <= kAnchorMaxSlots) {
+    if (anchored) {
+      go(k_step_ebe_multi<T, D, PT, DPN, true>);
+      return;
+    }
+  }
+  if (std::is_same<PT, PatKuhn24>::value && Gr.sparse)
+    go(k_step_ebe_multi<T, D, PT, DPN, false, true>);
+  else
+    go(k_step_ebe_multi<T, D, PT, DPN, false, false>);
+}
+
+// One batched step of every group of the element operator.
+template <typename T>
+void launch_step_ebe_multi(const bfe_plan_s& H, const EbeOp& E,
+                           const std::vector<EbeMultiLists>& ML, int32_t npw, int32_t S,
+                           const T* u1, const T* u2, T* u, const T* F, int32_t F_rows, int32_t t,
+                           int32_t q, T* hp, const IfaceRows<T>& IR) {
+  for (size_t g = 0; g < E.groups.size(); ++g) {
+    const EbeGroup& Gr = E.groups[g];
+#define BFE_EM(D_, PT_, DPN_)                                                                  \
+  launch_ebe_multi_pat<T, D_, PT_, DPN_>(H, E, Gr, ML[g], npw, S, u1, u2, u, F, F_rows, t, q, hp, \
+                                         IR)
+#define BFE_EM_D(PT_, DPN_) \
+  if (H.damped) BFE_EM(true, PT_, DPN_); else BFE_EM(false, PT_, DPN_);
+    switch (Gr.pattern * 10 + Gr.dpn) {
+      case PatKuhn24::ID * 10 + 3: BFE_EM_D(PatKuhn24, 3) break;
+      case PatTriRect::ID * 10 + 1: BFE_EM_D(PatTriRect, 1) break;
+      case PatTriRect::ID * 10 + 2: BFE_EM_D(PatTriRect, 2) break;
+      case PatTriRect::ID * 10 + 3: BFE_EM_D(PatTriRect, 3) break;
+      case PatTriIso::ID * 10 + 1: BFE_EM_D(PatTriIso, 1) break;
+      case PatTriIso::ID * 10 + 2: BFE_EM_D(PatTriIso, 2) break;
+      case PatTriIso::ID * 10 + 3: BFE_EM_D(PatTriIso, 3) break;
+      default: throw std::invalid_argument("no multi-source kernel for this element group");
+    }
+#undef BFE_EM_D
+#undef BFE_EM
+  }
+}
+
 template <typename T>
 void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_idx,
                  const int32_t* frc_src, int32_t p, const double* frcs, int32_t frc_rows,
@@ -2521,7 +2886,11 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   hipStream_t st = H.stream;
   const int32_t rpb = H.multi_rows;
   const int32_t rpw = H.multi_rpw;
-  const int32_t G = resolve_group<T>(H, rpw);
+  // the element operator's multi-source product when every group has a
+  // pattern kernel, else the row-grouped CSR operator
+  const EbeOp* E = (H.multi_ebe && ebe_multi_ok(H)) ? H.ebe.get() : nullptr;
+  H.last_multi_op = E ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
+  const int32_t G = E ? 0 : resolve_group<T>(H, rpw);
   H.last_group = G;
   const int64_t rows_per_wg = static_cast<int64_t>(rpb) * rpw;
   const int64_t nblk = (n + rows_per_wg - 1) / rows_per_wg;
@@ -2537,6 +2906,9 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   L.iptr = gi_ptr.as<int32_t>();
   L.i_row = gi_row.as<int32_t>();
   L.i_k = gi_k.as<int32_t>();
+  std::vector<EbeMultiLists> ML;
+  if (E)
+    ML = ebe_multi_lists<T>(H, *E, H.multi_npw, frc_idx, frc_src, p, frc_rows, hist_idx, q);
   DevBuf dhist(sizeof(T) * static_cast<size_t>(q) * S * n_t);
   DevBuf ubuf(sizeof(T) * static_cast<size_t>(n) * S * 3);
   BFE_HIP(hipMemsetAsync(ubuf.p, 0, ubuf.bytes, st));
@@ -2558,9 +2930,15 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     T* hp = (q > 0 && prev >= t0) ? hist + prev * per_t : nullptr;
     const int32_t ti = static_cast<int32_t>(t);
     sampler.before(t);
-    launch_multi<T>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp, IR);
+    if (E) {
+      launch_step_ebe_multi<T>(H, *E, ML, H.multi_npw, S, u1, u2, u, F, frc_rows, ti, q, hp, IR);
+      launches += static_cast<int64_t>(E->groups.size());
+    } else {
+      launch_multi<T>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp,
+                      IR);
+      ++launches;
+    }
     sampler.after(t);
-    ++launches;
     if (H.nI > 0 && H.has_xinv) {
       launch_iface<T>(H, S, wI, u);
       ++launches;
@@ -3157,6 +3535,19 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
       case BFE_TUNE_PERSISTENT:
         h->use_persistent = value ? 1 : 0;
         break;
+      case BFE_TUNE_MULTI_OPERATOR:
+        if (value != BFE_OPERATOR_CSR && value != BFE_OPERATOR_EBE)
+          throw std::invalid_argument("operator must be BFE_OPERATOR_CSR or BFE_OPERATOR_EBE");
+        h->multi_ebe = value == BFE_OPERATOR_EBE ? 1 : 0;
+        break;
+      case BFE_TUNE_MULTI_NODES_PER_WAVE:
+        if (value < 1 || value > 1024)
+          throw std::invalid_argument("nodes per wavefront must be in [1, 1024]");
+        h->multi_npw = value;
+        break;
+      case BFE_TUNE_MULTI_SLIDE:
+        h->multi_pipeline = value ? 1 : 0;
+        break;
       case BFE_TUNE_PERSISTENT_GRID:
         if (value < 0) throw std::invalid_argument("persistent grid cap must be >= 0");
         h->persistent_grid = value;
@@ -3215,6 +3606,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
       for (const auto& g : E.groups) I.ebe_sparse += g.sparse;
     }
     I.multi_group = h->last_group;
+    I.multi_operator = h->last_multi_op;
     *info = I;
   });
 }

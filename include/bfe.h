@@ -79,6 +79,7 @@ typedef struct bfe_info {
   int32_t multi_group;   /* rows per gather group of the last batched run */
   int32_t ebe_sparse;    /* element groups on a structurally sparse product
                             (zero template entries skipped; equal to rounding) */
+  int32_t multi_operator; /* operator of the last batched run: BFE_OPERATOR_* */
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
@@ -270,6 +271,23 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * With fewer workgroups than row blocks each workgroup sweeps blocks
  * lb, lb + grid, ... between the grid barriers; results do not change. */
 #define BFE_TUNE_PERSISTENT_GRID 10
+/* operator of the batched march (bfe_run_batched) once an element operator
+ * is attached: BFE_OPERATOR_EBE marches the multi-source element product
+ * (k_step_ebe_multi: one wavefront per node, lane = source, each wavefront
+ * marching a run of consecutive nodes) when every element group has a
+ * mesh-pattern kernel, else the row-grouped CSR operator is used;
+ * BFE_OPERATOR_CSR (the default: the faster of the two on the 2D FMC
+ * workload) always uses the CSR operator. bfe_info multi_operator reports
+ * the operator of the last batched run. Results agree to rounding. */
+#define BFE_TUNE_MULTI_OPERATOR 11
+/* consecutive nodes each wavefront of the multi-source element product
+ * marches (1 to 1024; default 4) */
+#define BFE_TUNE_MULTI_NODES_PER_WAVE 12
+/* 1 (default): along a run of regular nodes the multi-source element
+ * product keeps the neighbour rows that the next node shares in registers
+ * and gathers only the others; 0: every node gathers all its neighbour rows.
+ * Results are identical. */
+#define BFE_TUNE_MULTI_SLIDE 13
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3
@@ -302,7 +320,8 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value);
  * operators from the same random state and rejects the element operator
  * (BFE_ERR_ARG) if the results differ by more than check_tol relative
  * (<= 0: 1e-9 in double, 1e-3 in single precision). The CSR operator stays
- * in use for bfe_run_batched and the checkpointed/adjoint runs.
+ * in use for the checkpointed/adjoint runs (bfe_run_batched: see
+ * BFE_TUNE_MULTI_OPERATOR).
  * No counterpart in the reference (which always forms sparse B).
  */
 bfe_status bfe_set_element_operator(bfe_handle h, int64_t n_nodes, int32_t dpn, int64_t n_el,

@@ -7,7 +7,8 @@ import numpy as np
 from bristolfe_v2_amd import _lib, autotune
 
 KEYS = {_lib.TUNE_MULTI_ROWS_PER_BLOCK: "rows", _lib.TUNE_MULTI_ROWS_PER_WAVE: "rpw",
-        _lib.TUNE_XCD_REMAP: "remap", _lib.TUNE_MULTI_GROUP: "group"}
+        _lib.TUNE_XCD_REMAP: "remap", _lib.TUNE_MULTI_GROUP: "group",
+        _lib.TUNE_MULTI_OPERATOR: "op", _lib.TUNE_MULTI_NODES_PER_WAVE: "npw"}
 
 
 class SyntheticPlan:
@@ -16,7 +17,7 @@ class SyntheticPlan:
 
     def __init__(self, cost):
         self.cost = cost
-        self.shape = {"rows": 4, "rpw": 1, "remap": 1, "group": 0}
+        self.shape = {"rows": 4, "rpw": 1, "remap": 1, "group": 0, "op": 0, "npw": 4}
         self.requests = []
         self._ms = 0.0
 
@@ -56,7 +57,9 @@ def test_staged_search_finds_the_optimum_and_tunes_the_plan():
     best, table = autotune.tune_batched(plan, 64, fd, np.arange(64), np.ones((1, 400)), fd,
                                         probe_steps=20, repeats=1)
     assert (best["rows"], best["rpw"], best["group"], best["remap"], best["tile"]) == (2, 4, 4, 0, 64)
-    assert plan.shape == {"rows": 2, "rpw": 4, "remap": 0, "group": 4}
+    assert {k: plan.shape[k] for k in ("rows", "rpw", "remap", "group", "op")} == \
+        {"rows": 2, "rpw": 4, "remap": 0, "group": 4, "op": 0}
+    assert best["operator"] == "csr"   # no element operator on this plan
     assert min(r["ms_per_step"] for r in table) == best["ms_per_step"]
     # every request kept the group within the rows per wavefront
     assert all(r["group"] <= r["rpw"] for r in plan.requests)
@@ -115,3 +118,35 @@ def test_run_tiled_passes_each_pass_its_own_signal_rows():
     calls.clear()
     autotune.run_tiled(P(), n_t, n_src, fd, fs, F[:1], [0], 4)
     assert all(c[3].shape[0] == 1 for c in calls)
+
+
+def test_element_product_stage_when_an_element_operator_is_attached():
+    """With an element operator whose groups have pattern kernels, the last
+    stage times the multi-source element product at 2, 4 and 8 nodes per
+    wavefront, and keeps it when it is faster."""
+
+    class P(SyntheticPlan):
+        def info(self):
+            return {"multi_group": self.used_group(), "operator_kind": 1, "ebe_pattern": 2,
+                    "multi_operator": self.shape["op"]}
+
+    def cost(sh):
+        if sh.get("op"):
+            return 0.5 + 0.01 * abs(sh["npw"] - 4)
+        return _cost(sh)
+
+    class Q(P):
+        def run_batched(self, n_t, S, frc_dof, frc_src, frcs, hist_dof):
+            shape = dict(self.shape, group=self.used_group(), columns=S)
+            self.requests.append(shape)
+            self._ms = cost(shape) * (n_t - 1)
+            self._steps = n_t - 1
+            return np.zeros((len(hist_dof), S, n_t)), 1
+
+    plan = Q(_cost)
+    fd = np.arange(64)
+    best, table = autotune.tune_batched(plan, 64, fd, np.arange(64), np.ones((1, 400)), fd,
+                                        probe_steps=20, repeats=1)
+    assert best["operator"] == "ebe" and best["npw"] == 4
+    assert plan.shape["op"] == 1 and plan.shape["npw"] == 4
+    assert sorted(r["npw"] for r in table if r["operator"] == "ebe") == [2, 4, 8]

@@ -133,3 +133,102 @@ def test_batched_per_entry_signal_rows(mode):
                                                     solver_mode=mode, abd=(A, B, D))
         assert np.linalg.norm(r) > 0
         assert rel_l2(hb[:, k, :], r) < TOL
+
+
+def _ebe_case(name):
+    """(K, C, M, dt, t, op, sources, monitors) of a small model whose element
+    operator has a mesh-pattern kernel."""
+    from bristolfe_v2_amd import configs, ebe
+    m = PKG.mesh
+    if name == "coupled":
+        # two groups (CPE3 solid, AC2D3 water; a node -> first-DOF table),
+        # ASI2D2 interface rows, absorbing layer
+        mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3)
+        opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+        K, C, M, gl = assemble(mod, matls, opts)
+        src = gl[np.nonzero(gl[:, 3] > 0)[0][::5][:24], 3] - 1
+    elif name == "cube":
+        c = configs.cube_3d(el_size=1e-3, n_t=10)
+        mod, matls, opts = c["mod"], c["matls"], c["fe_options"]
+        K, C, M, gl = c["K"], c["C"], c["M"], c["gl_lookup"]
+        top = np.nonzero(np.isclose(mod["nds"][:, 2], mod["nds"][:, 2].max()))[0]
+        src = gl[top[::3][:20], 2] - 1
+    else:  # the union of disjoint plate meshes of the FMC workload, reduced
+        c = configs.fmc_multimodel(pitch=0.1e-3, n_sub=3, sub_dof=(4e3, 12e3), n_t=10,
+                                   plate=dict(width=8e-3, height=4e-3), sub_height=4e-3)
+        mod, matls, opts = c["mod"], c["matls"], c["fe_options"]
+        K, C, M, gl = c["K"], c["C"], c["M"], c["gl_lookup"]
+        src = c["frc_dof"][c["frc_src"] < 64][::4]
+    op = ebe.element_operator(mod, matls, opts, gl, K.shape[0])
+    assert op is not None
+    dt = m.stable_time_step(K, M.diagonal())
+    return K, C, M, dt, op, np.asarray(src), np.arange(0, K.shape[0], 13)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("name", ["union", "coupled", "cube"])
+def test_batched_element_product(name, mode):
+    """The multi-source element product (k_step_ebe_multi) against the CSR
+    multi-source step on the same plan (1e-13) and every 4th column against
+    the oracle's single-source run (1e-10): the union of disjoint plate meshes
+    (irregular nodes at the mesh seams), the coupled solid/water model (two
+    groups, a node -> DOF table, interface rows) and a C3D4 cube (stencil and
+    structurally sparse slot paths). Nodes per wavefront from 1 to 64, with
+    and without the sliding neighbour rows, give the same bits."""
+    print("case: batched element product", name, mode, flush=True)
+    from bristolfe_v2_amd import _lib
+    K, C, M, dt, op, src, hi = _ebe_case(name)
+    n_t = 200 if name != "cube" else 120
+    t = np.arange(n_t) * dt
+    s = PKG.mesh.hann_toneburst(t, 5e6 if name != "cube" else 2e6, 3)
+    S = src.size
+    plan = PKG.make_plan(K, C, M, dt, mode, element_op=op, operator="ebe")
+    try:
+        assert plan.info()["operator_kind"] == 1
+        hc, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+        assert plan.info()["multi_operator"] == 0   # CSR by default
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, 1)
+        he, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+        assert plan.info()["multi_operator"] == 1
+        assert np.linalg.norm(hc) > 0
+        assert rel_l2(he, hc) < 1e-13
+        for npw, slide in ((1, 1), (3, 1), (64, 1), (4, 0)):
+            plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, npw)
+            plan.set_tuning(_lib.TUNE_MULTI_SLIDE, slide)
+            h2, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+            np.testing.assert_array_equal(h2, he)
+    finally:
+        plan.close()
+    A, B, D = ov6.abd_matrices(K, C, M, dt, mode)
+    for k in range(0, S, 4):
+        r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, [src[k] + 1], s, hi + 1,
+                                                    solver_mode=mode, abd=(A, B, D))
+        assert rel_l2(he[:, k, :], r) < TOL
+
+
+def test_batched_element_product_single_precision():
+    """FP32 bulk march on the multi-source element product: error against the
+    FP64 oracle below 1e-3 (union of plate meshes, 16 sources)."""
+    print("case: batched element product fp32", flush=True)
+    from bristolfe_v2_amd import _lib
+    K, C, M, dt, op, src, hi = _ebe_case("union")
+    src = src[:16]
+    n_t = 200
+    t = np.arange(n_t) * dt
+    s = PKG.mesh.hann_toneburst(t, 5e6, 3)
+    plan = PKG.make_plan(K, C, M, dt, "vel at curent time step", "single", element_op=op,
+                         operator="ebe")
+    try:
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, 1)
+        h32, _ = plan.run_batched(n_t, src.size, src, np.arange(src.size), s, hi)
+        assert plan.info()["multi_operator"] == 1
+    finally:
+        plan.close()
+    A, B, D = ov6.abd_matrices(K, C, M, dt, "vel at curent time step")
+    worst = 0.0
+    for k in (0, src.size - 1):
+        r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, [src[k] + 1], s, hi + 1,
+                                                    abd=(A, B, D))
+        worst = max(worst, rel_l2(h32[:, k, :], r))
+    print(f"batched element product fp32 rel L2 vs fp64 oracle: {worst:.3e}")
+    assert worst < TOL_FP32
