@@ -87,3 +87,27 @@ def build_global_matrices(nds, els, el_mat_i, el_abs_i, el_typ_i, matls, fe_opti
     for new, g in enumerate(idx):
         gl[g // max_df, g % max_df] = new + 1
     return Kg, Cg, Mg, gl
+
+
+def add_interface_damping(els, el_typ_i, K, C, M, gl_lookup, factor=1.0):
+    """code/internal/fn_add_interface_damping.m:18-53 on dense arrays, line by
+    line: level = min(sqrt(diag K ./ diag M)) / pi^2 * factor (:30, NaN
+    skipped as MATLAB's min does), gi = the interface nodes' global indices
+    in MATLAB's column-major order, duplicates kept (:33-45), and
+    C(gi, gi) = C(gi, gi) + M(gi, gi) * level (:52), whose duplicate
+    positions are assigned in order, the last one kept (numpy's fancy
+    assignment does the same). Returns a dense C."""
+    Kd, Cd, Md = (np.asarray(sp.csr_matrix(A).toarray()) for A in (K, C, M))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        level = np.nanmin(np.sqrt(np.diag(Kd) / np.diag(Md))) / np.pi ** 2 * factor
+    typ = np.asarray(el_typ_i, dtype=object).ravel()
+    nds = np.asarray(els)[typ == "ASI2D2", :]
+    nds = nds.ravel(order="F")
+    nds = nds[nds > 0]
+    if nds.size == 0:
+        return Cd
+    gi = np.asarray(gl_lookup)[nds - 1, :].ravel(order="F")
+    gi = gi[gi > 0] - 1
+    ix = np.ix_(gi, gi)
+    Cd[ix] = Cd[ix] + Md[ix] * level
+    return Cd

@@ -217,3 +217,70 @@ def test_tet_mesh_orientation_and_volume():
     assert np.all(J > 0)
     np.testing.assert_allclose(J.sum() / 6, 2e-3 * 1e-3 * 1.5e-3, rtol=1e-12)
     assert mod["els"].shape[0] == 6 * 4 * 2 * 3
+
+
+def test_interface_damping_matches_the_reference_lines():
+    """fn_add_interface_damping.m:18-53: the product's diagonal form against
+    the oracle's dense, line-by-line form (duplicate interface nodes included,
+    as MATLAB's C(gi, gi) assignment sees them), plus what the formula fixes:
+    only the interface nodes' DOFs change, each once, by level * M_ii with
+    level = min sqrt(K_ii / M_ii) / pi^2 * factor; linear in the factor; a
+    model without interface elements is left as it is."""
+    from oracle import assembly as oa
+    mod, matls = coupled_block(1.5e-3, 0.75e-3, 0.5e-3, 0.25e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
+    typ = np.asarray(mod["el_typ_i"], dtype=object)
+    asi = np.asarray(mod["els"])[typ == "ASI2D2"]
+    assert asi.shape[0] >= 2  # neighbouring interface elements share a node
+    for factor in (1.0, 0.3):
+        Cp = PKG.assembly.fn_add_interface_damping(mod["els"], typ, K, C, M, gl, factor)
+        Co = oa.add_interface_damping(mod["els"], typ, K, C, M, gl, factor)
+        np.testing.assert_allclose(Cp.toarray(), Co, rtol=1e-15, atol=0)
+    dK, dM = K.diagonal(), M.diagonal()
+    level = np.min(np.sqrt(dK / dM)) / np.pi ** 2
+    nodes = np.unique(asi[asi > 0])
+    gi = gl[nodes - 1].ravel()
+    gi = np.unique(gi[gi > 0]) - 1
+    D = (PKG.assembly.fn_add_interface_damping(mod["els"], typ, K, C, M, gl, 2.0) - C).toarray()
+    expect = np.zeros(K.shape[0])
+    expect[gi] = 2.0 * level * dM[gi]
+    np.testing.assert_allclose(np.diag(D), expect, rtol=1e-14, atol=0)
+    assert not (D - np.diag(np.diag(D))).any()
+    # a solid-only model has no interface elements: C unchanged
+    mod2, matls2 = solid_plate(1e-3, 0.5e-3, 0.25e-3)
+    K2, C2, M2, gl2 = assemble(mod2, matls2, {"dof_to_use": [1, 2]})
+    typ2 = np.asarray(["CPE3"] * len(mod2["els"]), dtype=object)
+    C2b = PKG.assembly.fn_add_interface_damping(mod2["els"], typ2, K2, C2, M2, gl2)
+    assert (C2b != C2).nnz == 0
+
+
+@pytest.mark.parametrize("which", ["u", "v"])
+def test_div_curl_closed_forms_on_one_element(which):
+    """fn_get_field_output.m:38-58, 85-105 on one equilateral CPE3 triangle
+    (centroid at c, vertices at distance R from it): a rigid translation gives
+    div = curl = 0 (the unit vectors from the centroid sum to zero); the
+    radial field u = x - c gives div = 3R, curl = 0; the rotation
+    u = (-(y - cy), x - cx) gives div = 0, curl = -3R (the curl unit vectors
+    are the outward ones turned by -90 degrees); and both are linear."""
+    R = 0.4e-3
+    ang = np.deg2rad([90.0, 210.0, 330.0])
+    c = np.array([1e-3, 2e-3])
+    nds = c + R * np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    mod = {"nds": nds, "els": np.array([[1, 2, 3]]), "el_mat_i": np.array([1]),
+           "el_abs_i": np.zeros(1), "el_typ_i": np.array(["CPE3"], dtype=object)}
+    K, C, M, gl = assemble(mod, [steel()], {"dof_to_use": [1, 2]})
+    d = nds - c
+    f = np.zeros((K.shape[0], 4))
+    for k in range(3):
+        ix, iy = gl[k, 0] - 1, gl[k, 1] - 1
+        f[ix, 0], f[iy, 0] = 0.7, -0.2                 # translation
+        f[ix, 1], f[iy, 1] = d[k, 0], d[k, 1]          # expansion
+        f[ix, 2], f[iy, 2] = -d[k, 1], d[k, 0]         # rotation
+    f[:, 3] = 2.0 * f[:, 1] - 3.0 * f[:, 2]
+    mats = {"M": M, "gl_lookup": gl}
+    div = PKG.fn_get_field_output(f, mod, mats, f"div({which})")
+    curl = PKG.fn_get_field_output(f, mod, mats, f"curl({which})")
+    # rounding: about eps times the field's size (0.7 for the translation, R else)
+    atol = np.array([4e-15, 1e-15 * R, 1e-15 * R, 1e-15 * R])
+    for got, want in ((div[0], [0.0, 3 * R, 0.0, 6 * R]), (curl[0], [0.0, 0.0, -3 * R, 9 * R])):
+        assert np.all(np.abs(got - np.array(want)) <= atol + 1e-14 * np.abs(want)), got
