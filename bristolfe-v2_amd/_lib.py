@@ -36,7 +36,7 @@ TUNE_MULTI_SLIDE = 13
 # Every symbol the header declares.
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
-    "bfe_run_checkpointed", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
+    "bfe_run_checkpointed", "bfe_run_checkpointed_batched", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
     "bfe_set_element_groups", "bfe_set_field_elements", "bfe_run_field_ke",
     "bfe_set_kernel_timing",
     "bfe_info_get",
@@ -119,6 +119,10 @@ def load() -> C.CDLL:
     lib.bfe_run_checkpointed.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i32p, C.c_int32, _f64p,
                                          C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32, _i32p]
     lib.bfe_run_checkpointed.restype = C.c_int
+    lib.bfe_run_checkpointed_batched.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                                 _i32p, _i32p, C.c_int32, _f64p, C.c_int32, _i32p,
+                                                 C.c_int32, _f64p, C.c_int32, _i32p]
+    lib.bfe_run_checkpointed_batched.restype = C.c_int
     lib.bfe_adjoint.argtypes = [C.c_void_p, C.c_int32, _f64p, _f64p, _f64p]
     lib.bfe_adjoint.restype = C.c_int
     lib.bfe_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
@@ -383,6 +387,27 @@ class Plan:
         check(self._lib.bfe_run_checkpointed(
             self._h, int(slot), int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p),
             rows, ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(ckpt_every),
+            C.byref(ti)))
+        return hist, int(ti.value)
+
+    def run_checkpointed_batched(self, slot, n_t, n_src, frc_idx, frc_src, frcs, hist_idx,
+                                 ckpt_every):
+        """bfe_run_checkpointed_batched; returns hist (q, n_src, n_t) and ti_start."""
+        frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
+        frc_src = np.ascontiguousarray(frc_src, np.int32).ravel()
+        hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
+        p, q = frc_idx.size, hist_idx.size
+        frcs = np.asarray(frcs, np.float64)
+        if frcs.ndim == 1:
+            frcs = frcs[None, :]
+        rows = frcs.shape[0] if p > 0 else 0
+        Fcol = np.asfortranarray(frcs) if p > 0 else np.zeros((1, 1))
+        hist = np.empty((q, n_src, n_t), order="F")  # the library writes every entry
+        ti = C.c_int32()
+        check(self._lib.bfe_run_checkpointed_batched(
+            self._h, int(slot), int(n_t), int(n_src), ptr(frc_idx, C.c_int32),
+            ptr(frc_src, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
+            ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(ckpt_every),
             C.byref(ti)))
         return hist, int(ti.value)
 

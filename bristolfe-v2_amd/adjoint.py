@@ -16,9 +16,10 @@ mass (:115). At s = 1, rho = rho0 the matrices are exactly the assembled
 ones. Parameter elements are the solid ones (CPE3, C3D4).
 
 The march and its adjoint run in libbfe_hip.so: ``bfe_run_checkpointed``
-keeps checkpoints every ``ckpt_every`` steps, ``bfe_adjoint`` marches the
-adjoint backward (fused with the recomputation of the checkpointed segments)
-and accumulates the gradients in FP64.
+(one transmitter) or ``bfe_run_checkpointed_batched`` (up to 32 transmitters
+in one multi-source march) keeps checkpoints every ``ckpt_every`` steps,
+``bfe_adjoint`` marches the adjoint backward (fused with the recomputation
+of the checkpointed segments) and accumulates the gradients in FP64.
 
 ``MarchFunction`` wraps this as a torch.autograd.Function on CPU float64
 tensors (torch only does the bookkeeping; the GPU work is in the library).
@@ -171,6 +172,51 @@ def forward(model: ParametrizedModel, shots: Shots, s=None, rho=None, keep=False
     return out
 
 
+def _entry_signals(shots: Shots, txs):
+    """Signal rows of a multi-source march over the transmitters txs: the one
+    shared row, or, when every transmitter forces one DOF per signal row, the
+    rows repeated for each transmitter (the forcing entries' order)."""
+    sig = shots.signal
+    if sig.shape[0] == 1:
+        return sig
+    if any(t.size != sig.shape[0] for t in txs):
+        raise ValueError("signal rows must be 1 or one per forcing DOF of every transmitter")
+    return np.concatenate([sig] * len(txs))
+
+
+def forward_batched(model: ParametrizedModel, shots: Shots, s=None, rho=None, keep=False):
+    """A-scans (n_tx, n_rx, n_t) of all transmitters (at most 32) in one
+    checkpointed multi-source march. With keep=True also returns the plan
+    holding the checkpoints (slot 0) for backward_batched and the HIP-event
+    loop time."""
+    if len(shots.tx) > 32:
+        raise ValueError("the batched adjoint takes at most 32 transmitters")
+    plan = model.plan(shots.dt, s, rho)
+    fi = np.concatenate(shots.tx)
+    fs = np.concatenate([np.full(t.size, k) for k, t in enumerate(shots.tx)])
+    sig = _entry_signals(shots, shots.tx)
+    try:
+        h, _ = plan.run_checkpointed_batched(0, shots.n_t, len(shots.tx), fi, fs, sig, shots.rx,
+                                             shots.ckpt_every)
+        ms = plan.stats()["loop_ms"]
+    except Exception:
+        plan.close()
+        raise
+    out = np.ascontiguousarray(np.transpose(h, (1, 0, 2)))
+    if keep:
+        return out, plan, ms
+    plan.close()
+    return out
+
+
+def backward_batched(plan, grad_ascans):
+    """dJ/ds, dJ/drho from dJ/dA-scans (n_tx, n_rx, n_t) of a forward_batched
+    plan (consumes its checkpoints): one reverse sweep for all transmitters."""
+    g = np.asfortranarray(np.transpose(np.asarray(grad_ascans, np.float64), (1, 0, 2)))
+    gk, gm = plan.adjoint(0, g)
+    return gk, gm, plan.stats()["loop_ms"]
+
+
 def ascans_batched(model: ParametrizedModel, shots: Shots, s=None, rho=None):
     """A-scans (n_tx, n_rx, n_t) of all transmitters in one multi-source march
     (no checkpoints): the cheap evaluator for finite differences."""
@@ -183,7 +229,8 @@ def ascans_batched(model: ParametrizedModel, shots: Shots, s=None, rho=None):
         fs = np.concatenate([np.full(t.size, k) for k, t in enumerate(grp)])
         with _lib.Plan.from_scipy(K, np.asarray(M.sum(axis=0)).ravel(), C, shots.dt,
                                   SOLVER_MODES[o["solver_mode"]], _precision(o["solver_precision"])) as p:
-            h, _ = p.run_batched(shots.n_t, len(grp), fi, fs, shots.signal, shots.rx)
+            h, _ = p.run_batched(shots.n_t, len(grp), fi, fs, _entry_signals(shots, grp),
+                                 shots.rx)
         out[c0:c0 + len(grp)] = np.transpose(h, (1, 0, 2))
     return out
 
@@ -213,16 +260,21 @@ def march_function():
 
         @staticmethod
         def forward(ctx, s, rho, model, shots):
-            a, plan, _ = forward(model, shots, s.detach().cpu().numpy(), rho.detach().cpu().numpy(),
-                                 keep=True)
+            # all transmitters in one checkpointed multi-source march when
+            # they fit one (at most 32), else one march per transmitter
+            ctx.batched = len(shots.tx) <= 32
+            fwd = forward_batched if ctx.batched else forward
+            a, plan, _ = fwd(model, shots, s.detach().cpu().numpy(), rho.detach().cpu().numpy(),
+                             keep=True)
             ctx.plan = plan
             return torch.from_numpy(a)
 
         @staticmethod
         def backward(ctx, grad):
             plan = ctx.plan
+            bwd = backward_batched if ctx.batched else backward
             try:
-                gk, gm, _ = backward(plan, grad.detach().cpu().numpy().astype(np.float64))
+                gk, gm, _ = bwd(plan, grad.detach().cpu().numpy().astype(np.float64))
             finally:
                 plan.close()
                 ctx.plan = None

@@ -1268,6 +1268,81 @@ __global__ __launch_bounds__(kBlock) void k_grad(
   gM[e] -= sm;
 }
 
+// k_grad over S sources at once: one thread per (element, source), GS
+// (a power of two >= S) consecutive lanes per element, so the lanes of an
+// element read the S sources of each DOF as one contiguous run. State
+// (t, dof, s) of the forward march sits at U[t * us + dof * ud + s], of the
+// adjoint at Lam[t * ls + dof * ld + s] (ls may be negative: the adjoint is
+// stored in reverse time); the states before the segment are n x S. The
+// element's sums over its sources are reduced across the GS lanes and added
+// by the first lane. (Splitting the time loop into blocks of 16 steps, with
+// consecutive workgroups on one block, ran 27.0 -> 28.2 ms per launch on the
+// 16-transmitter adjoint workload: not kept.)
+template <typename T, int KMAX>
+__global__ __launch_bounds__(kBlock) void k_grad_multi(
+    int32_t n_el, int32_t S, int32_t GS, const int32_t* __restrict__ el_dofs,
+    const T* __restrict__ el_K, const T* __restrict__ el_m, const T* __restrict__ el_w,
+    const T* __restrict__ U, int64_t us, int64_t ud, const T* __restrict__ Lam, int64_t ls,
+    int64_t ld, int32_t Ls, const T* __restrict__ um1, const T* __restrict__ um2, double dt2,
+    double* __restrict__ gK, double* __restrict__ gM) {
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t e = tid / GS;
+  const int s = static_cast<int>(tid - e * GS);
+  const bool on = e < n_el && s < S;
+  const int64_t ee = e < n_el ? e : 0;  // threads past the end read element 0, then drop it
+  const int ss = s < S ? s : 0;
+  int32_t d[KMAX];
+  T Ke[KMAX * KMAX], me[KMAX], p1[KMAX], p2[KMAX];
+#pragma unroll
+  for (int i = 0; i < KMAX; ++i) {
+    d[i] = el_dofs[ee * KMAX + i];
+    me[i] = el_m[ee * KMAX + i];
+    const int64_t di = d[i] >= 0 ? d[i] : 0;
+    const T a1 = um1[di * S + ss], a2 = um2[di * S + ss];
+    p1[i] = d[i] >= 0 ? a1 : T(0);
+    p2[i] = d[i] >= 0 ? a2 : T(0);
+  }
+#pragma unroll
+  for (int i = 0; i < KMAX * KMAX; ++i) Ke[i] = el_K[ee * KMAX * KMAX + i];
+  const T w0 = el_w[3 * ee], w1 = el_w[3 * ee + 1], w2 = el_w[3 * ee + 2];
+  double sk = 0.0, sm = 0.0;
+  for (int32_t t = 0; t < Ls; ++t) {
+    T ut[KMAX], lt[KMAX];
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      const int64_t di = d[i] >= 0 ? d[i] : 0;
+      const T x = U[t * us + di * ud + ss], y = Lam[t * ls + di * ld + ss];
+      ut[i] = d[i] >= 0 ? x : T(0);
+      lt[i] = d[i] >= 0 ? y : T(0);
+    }
+    T ks = T(0), ms = T(0);
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      T r = T(0);
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j) r += Ke[i * KMAX + j] * p1[j];
+      ks += lt[i] * r;
+      ms += lt[i] * me[i] * (w0 * ut[i] + w1 * p1[i] + w2 * p2[i]);
+    }
+    sk += static_cast<double>(ks);
+    sm += static_cast<double>(ms);
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      p2[i] = p1[i];
+      p1[i] = ut[i];
+    }
+  }
+  if (!on) sk = sm = 0.0;
+  for (int o = GS >> 1; o > 0; o >>= 1) {  // the element's lanes, in one wavefront
+    sk += __shfl_xor(sk, o, 64);
+    sm += __shfl_xor(sm, o, 64);
+  }
+  if (s == 0 && e < n_el) {
+    gK[e] -= dt2 * sk;
+    gM[e] -= sm;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Device buffers owned by a handle or a run.
 struct DevBuf {
@@ -2037,6 +2112,11 @@ struct AdjointRecord {
   DevBuf ckpt;            // per segment k: u(start_k - 1), u(start_k - 2)
   DevBuf useg[2];         // C x n forward states of one segment each
   int cur = 0;            // useg[cur] holds the last segment after the forward
+  // batched record (bfe_run_checkpointed_batched): S sources, states n x S
+  // (source axis contiguous); useg[0] holds the last segment's C x n x S
+  bool batched = false;
+  int32_t S = 1;
+  std::vector<int32_t> frc_src;
 };
 
 template <typename T>
@@ -2541,6 +2621,33 @@ void launch_grad(const bfe_plan_s& H, const GradElements& G, const T* U, const T
 // column 1 of a two-column march while column 0 recomputes segment k-1 from
 // its checkpoint; then the gradient kernel consumes segment k.
 template <typename T>
+void launch_grad_multi(const bfe_plan_s& H, const GradElements& G, int32_t S, const T* U,
+                       int64_t us, int64_t ud, const T* Lam, int64_t ls, int64_t ld, int32_t Ls,
+                       const T* um1, const T* um2, double* gK, double* gM) {
+  int32_t GS = 1;
+  while (GS < S) GS <<= 1;
+  const int64_t threads = static_cast<int64_t>(G.n_el) * GS;
+  const dim3 grid((threads + kBlock - 1) / kBlock), blk(kBlock);
+  const double dt2 = H.dt * H.dt;
+  const int32_t* d = G.dofs.as<int32_t>();
+  const T *Kp = G.K.as<T>(), *mp = G.m.as<T>(), *wp = G.w.as<T>();
+#define BFE_GRADM(KM)                                                                          \
+  hipLaunchKernelGGL((k_grad_multi<T, KM>), grid, blk, 0, H.stream, G.n_el, S, GS, d, Kp, mp, wp, \
+                     U, us, ud, Lam, ls, ld, Ls, um1, um2, dt2, gK, gM)
+  switch (G.kmax) {
+    case 2: BFE_GRADM(2); break;
+    case 3: BFE_GRADM(3); break;
+    case 4: BFE_GRADM(4); break;
+    case 6: BFE_GRADM(6); break;
+    case 8: BFE_GRADM(8); break;
+    case 9: BFE_GRADM(9); break;
+    case 12: BFE_GRADM(12); break;
+    default: throw std::invalid_argument("unsupported element DOF count for gradients");
+  }
+#undef BFE_GRADM
+}
+
+template <typename T>
 void run_adjoint(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
                  const double* hist_grad, double* g_stiff, double* g_dens) {
   const int64_t n = H.n;
@@ -2867,7 +2974,7 @@ template <typename T>
 void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_idx,
                  const int32_t* frc_src, int32_t p, const double* frcs, int32_t frc_rows,
                  const int32_t* hist_idx, int32_t q, double* hist_out,
-                 int32_t* ti_start_out) {
+                 int32_t* ti_start_out, AdjointRecord* rec = nullptr, int32_t ckpt_every = 0) {
   const int64_t n = H.n;
   validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   if (n_src < 1 || n_src > 64) throw std::invalid_argument("n_src must be in [1, 64]");
@@ -2877,6 +2984,21 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   H.stats.columns = S;
   const int64_t t0 = p > 0 ? first_forced_step(frcs, frc_rows, n_t) : -1;
   if (ti_start_out) *ti_start_out = t0 < 0 ? n_t + 1 : static_cast<int32_t>(t0 + 1);
+  if (rec) {
+    *rec = AdjointRecord{};
+    rec->n_t = n_t;
+    rec->p = p;
+    rec->frc_rows = frc_rows;
+    rec->q = q;
+    rec->t0 = t0;
+    rec->batched = true;
+    rec->S = S;
+    rec->frc_idx.assign(frc_idx, frc_idx + p);
+    rec->frc_src.assign(frc_src, frc_src + p);
+    rec->hist_idx.assign(hist_idx, hist_idx + q);
+    if (p > 0) rec->F.assign(frcs, frcs + static_cast<size_t>(frc_rows) * n_t);
+    rec->valid = true;
+  }
   if (t0 < 0) {
     if (q > 0) std::memset(hist_out, 0, sizeof(double) * static_cast<size_t>(q) * S * n_t);
     return;
@@ -2887,8 +3009,9 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   const int32_t rpb = H.multi_rows;
   const int32_t rpw = H.multi_rpw;
   // the element operator's multi-source product when every group has a
-  // pattern kernel, else the row-grouped CSR operator
-  const EbeOp* E = (H.multi_ebe && ebe_multi_ok(H)) ? H.ebe.get() : nullptr;
+  // pattern kernel, else the row-grouped CSR operator (a checkpointed run
+  // stays on the CSR operator that the adjoint's recomputation uses)
+  const EbeOp* E = (H.multi_ebe && rec == nullptr && ebe_multi_ok(H)) ? H.ebe.get() : nullptr;
   H.last_multi_op = E ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
   const int32_t G = E ? 0 : resolve_group<T>(H, rpw);
   H.last_group = G;
@@ -2922,6 +3045,12 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   const T* F = R.F.as<T>();
   const int64_t per_t = static_cast<int64_t>(q) * S;
   int64_t launches = 0;
+  int64_t next_seg = 0;
+  if (rec) {
+    rec->seg = make_segments(t0, n_t, ckpt_every);
+    rec->ckpt = DevBuf(sizeof(T) * static_cast<size_t>(ns) * 2 * rec->seg.count());
+    rec->useg[0] = DevBuf(sizeof(T) * static_cast<size_t>(ns) * rec->seg.C);
+  }
   KernelSampler sampler(H, t0, n_t - t0);
 
   BFE_HIP(hipEventRecord(H.ev0, st));
@@ -2929,6 +3058,13 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     const int64_t prev = t - 1;
     T* hp = (q > 0 && prev >= t0) ? hist + prev * per_t : nullptr;
     const int32_t ti = static_cast<int32_t>(t);
+    if (rec && next_seg < rec->seg.count() && t == rec->seg.start[next_seg]) {
+      // checkpoint: the two states the step at t starts from
+      T* ck = rec->ckpt.as<T>() + next_seg * 2 * ns;
+      BFE_HIP(hipMemcpyAsync(ck, u1, sizeof(T) * ns, hipMemcpyDeviceToDevice, st));
+      BFE_HIP(hipMemcpyAsync(ck + ns, u2, sizeof(T) * ns, hipMemcpyDeviceToDevice, st));
+      ++next_seg;
+    }
     sampler.before(t);
     if (E) {
       launch_step_ebe_multi<T>(H, *E, ML, H.multi_npw, S, u1, u2, u, F, frc_rows, ti, q, hp, IR);
@@ -2943,6 +3079,12 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
       launch_iface<T>(H, S, wI, u);
       ++launches;
     }
+    if (rec) {  // every state of the last segment
+      const int64_t last = rec->seg.start[rec->seg.count() - 1];
+      if (t >= last)
+        BFE_HIP(hipMemcpyAsync(rec->useg[0].as<T>() + (t - last) * ns, u, sizeof(T) * ns,
+                               hipMemcpyDeviceToDevice, st));
+    }
     T* tmp = u2; u2 = u1; u1 = u; u = tmp;
   }
   if (q > 0) {
@@ -2953,6 +3095,174 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   }
   finish_timing(H, sampler, n_t - t0, launches);
   if (q > 0) H.stats.nonfinite = copy_out<T>(dhist, hist_out, static_cast<size_t>(per_t) * n_t);
+}
+
+// The reverse sweep of a batched checkpointed run (S sources): one
+// multi-source march of 2S columns per step, columns 0..S-1 recomputing
+// segment k-1 from its checkpoint and columns S..2S-1 marching the adjoint
+// over segment k (the sources' receiver residuals as forcing), so the
+// operator is read once for both. Each sweep writes its states straight
+// into a C x n x 2S buffer (no copies); two buffers alternate, because the
+// gradient of segment k pairs the forward states recomputed in sweep k + 1
+// with the adjoint states of sweep k. The gradients are summed over the
+// sources by k_grad_multi.
+template <typename T>
+void run_adjoint_batched(bfe_plan_s& H, AdjointRecord& rec, const GradElements& G,
+                         const double* hist_grad, double* g_stiff, double* g_dens) {
+  const int64_t n = H.n;
+  const int32_t S = rec.S, S2 = 2 * rec.S;
+  if (S2 > 64) throw std::invalid_argument("a batched adjoint takes at most 32 sources");
+  std::memset(g_stiff, 0, sizeof(double) * G.n_el);
+  std::memset(g_dens, 0, sizeof(double) * G.n_el);
+  H.stats = bfe_run_stats{};
+  H.stats.columns = S2;
+  if (rec.t0 < 0) return;
+  BFE_HIP(hipSetDevice(H.device));
+  hipStream_t st = H.stream;
+  const int32_t n_t = rec.n_t;
+  const Segments& sg = rec.seg;
+  const int64_t nseg = sg.count(), C = sg.C;
+  const double dt2 = H.dt * H.dt;
+  const int32_t q = rec.q;
+
+  // adjoint forcing: receivers with duplicate DOFs summed, per source, / dt^2
+  std::vector<int32_t> rdof;
+  std::vector<int32_t> rrow(q);
+  {
+    std::vector<std::pair<int32_t, int32_t>> kk;
+    for (int32_t k = 0; k < q; ++k) kk.push_back({rec.hist_idx[k], k});
+    std::sort(kk.begin(), kk.end());
+    for (auto& e : kk) {
+      if (rdof.empty() || rdof.back() != e.first) rdof.push_back(e.first);
+      rrow[e.second] = static_cast<int32_t>(rdof.size()) - 1;
+    }
+  }
+  const int32_t qa = static_cast<int32_t>(rdof.size());
+  // Fadj (t, receiver, source); hist_grad element (k, s, t) at k + q (s + S t)
+  std::vector<double> Fadj(static_cast<size_t>(qa) * S * n_t, 0.0);
+  for (int64_t t = 0; t < n_t; ++t)
+    for (int32_t s = 0; s < S; ++s)
+      for (int32_t k = 0; k < q; ++k)
+        Fadj[(t * S + s) * qa + rrow[k]] += hist_grad[k + static_cast<int64_t>(q) * (s + S * t)] / dt2;
+
+  // forcing entries of the 2S-column march: the forward entries in their
+  // source's column, then one adjoint entry per (receiver, source)
+  const int32_t pf = rec.p, frow_f = rec.frc_rows;
+  std::vector<int32_t> fi, fs;
+  for (int32_t i = 0; i < pf; ++i) { fi.push_back(rec.frc_idx[i]); fs.push_back(rec.frc_src[i]); }
+  for (int32_t s = 0; s < S; ++s)
+    for (int32_t k = 0; k < qa; ++k) { fi.push_back(rdof[k]); fs.push_back(S + s); }
+  const int32_t ptot = static_cast<int32_t>(fi.size());
+  const int32_t rpb = H.multi_rows, rpw = H.multi_rpw;
+  const int32_t Gg = resolve_group<T>(H, rpw);
+  const int64_t rows_per_wg = static_cast<int64_t>(rpb) * rpw;
+  const int64_t nblk = (n + rows_per_wg - 1) / rows_per_wg;
+  std::vector<double> Fentry(static_cast<size_t>(ptot) * C, 0.0);  // one row per entry
+  RunBuffers R;
+  make_run_buffers<T>(R, H, rows_per_wg, nblk, S2, fi.data(), fs.data(), ptot, Fentry.data(),
+                      ptot, static_cast<int32_t>(C), nullptr, 0);
+  BlockLists L = R.lists(H);
+  std::vector<int32_t> ks(H.iface_host.size());
+  for (size_t k = 0; k < ks.size(); ++k) ks[k] = static_cast<int32_t>(k);
+  Grouped gi = group_by_block(nblk, rows_per_wg, H.iface_host, &ks, nullptr);
+  DevBuf gi_ptr = upload(gi.ptr), gi_row = upload(gi.row), gi_k = upload(gi.a);
+  L.iptr = gi_ptr.as<int32_t>();
+  L.i_row = gi_row.as<int32_t>();
+  L.i_k = gi_k.as<int32_t>();
+  T* wI = R.wI.as<T>();
+  const IfaceRows<T> IR = iface_rows<T>(H, wI);
+  const T* falpha = R.falpha.as<T>();
+  const int64_t nS = n * S, nS2 = n * S2;
+  DevBuf buf[2], init[2];
+  for (int b = 0; b < 2; ++b) {
+    buf[b] = DevBuf(sizeof(T) * static_cast<size_t>(nS2) * C);
+    init[b] = DevBuf(sizeof(T) * static_cast<size_t>(nS2) * 2);  // the two states before a sweep
+  }
+  DevBuf dgK(sizeof(double) * G.n_el), dgM(sizeof(double) * G.n_el);
+  BFE_HIP(hipMemsetAsync(dgK.p, 0, dgK.bytes, st));
+  BFE_HIP(hipMemsetAsync(dgM.p, 0, dgM.bytes, st));
+  int64_t launches = 0;
+  std::vector<T> Fj(static_cast<size_t>(ptot) * C);
+  KernelSampler sampler(H, 0, 0);
+  const size_t col_bytes = sizeof(T) * S;  // one row's S columns
+
+  BFE_HIP(hipEventRecord(H.ev0, st));
+  const T* last1 = nullptr;  // the adjoint states the previous sweep ended with
+  const T* last2 = nullptr;
+  for (int64_t k = nseg - 1; k >= 0; --k) {
+    const int64_t La = sg.len(k);
+    const int64_t Lf = k >= 1 ? sg.len(k - 1) : 0;
+    const int64_t a_end = sg.start[k + 1];        // adjoint walks t = a_end-1 .. start[k]
+    const int64_t f_beg = k >= 1 ? sg.start[k - 1] : 0;
+    T* bk = buf[k & 1].as<T>();
+    T* i1 = init[k & 1].as<T>();
+    T* i2 = i1 + nS2;
+    // the forward columns restart from checkpoint k-1 (zero before the first
+    // segment); the adjoint columns continue from the previous sweep
+    if (k >= 1) {
+      const T* ck = rec.ckpt.as<T>() + (k - 1) * 2 * nS;
+      BFE_HIP(hipMemcpy2DAsync(i1, sizeof(T) * S2, ck, col_bytes, col_bytes, n,
+                               hipMemcpyDeviceToDevice, st));
+      BFE_HIP(hipMemcpy2DAsync(i2, sizeof(T) * S2, ck + nS, col_bytes, col_bytes, n,
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+      BFE_HIP(hipMemset2DAsync(i1, sizeof(T) * S2, 0, col_bytes, n, st));
+      BFE_HIP(hipMemset2DAsync(i2, sizeof(T) * S2, 0, col_bytes, n, st));
+    }
+    if (last1 != nullptr) {
+      BFE_HIP(hipMemcpy2DAsync(i1 + S, sizeof(T) * S2, last1 + S, sizeof(T) * S2, col_bytes, n,
+                               hipMemcpyDeviceToDevice, st));
+      BFE_HIP(hipMemcpy2DAsync(i2 + S, sizeof(T) * S2, last2 + S, sizeof(T) * S2, col_bytes, n,
+                               hipMemcpyDeviceToDevice, st));
+    } else {
+      BFE_HIP(hipMemset2DAsync(i1 + S, sizeof(T) * S2, 0, col_bytes, n, st));
+      BFE_HIP(hipMemset2DAsync(i2 + S, sizeof(T) * S2, 0, col_bytes, n, st));
+    }
+    const int64_t steps = std::max(La, Lf);
+    // signals of this sweep, one row per forcing entry, column = sweep step
+    for (int64_t j = 0; j < steps; ++j) {
+      for (int32_t i = 0; i < pf; ++i)
+        Fj[j * ptot + i] = (j < Lf) ? static_cast<T>(rec.F[(f_beg + j) * frow_f + (frow_f == 1 ? 0 : i)])
+                                    : T(0);
+      for (int32_t s = 0; s < S; ++s)
+        for (int32_t r = 0; r < qa; ++r)
+          Fj[j * ptot + pf + s * qa + r] =
+              (j < La) ? static_cast<T>(Fadj[((a_end - 1 - j) * S + s) * qa + r]) : T(0);
+    }
+    BFE_HIP(hipMemcpyAsync(R.F.p, Fj.data(), sizeof(T) * static_cast<size_t>(ptot) * steps,
+                           hipMemcpyHostToDevice, st));
+    for (int64_t j = 0; j < steps; ++j) {
+      T* u = bk + j * nS2;
+      const T* u1 = j >= 1 ? bk + (j - 1) * nS2 : i1;
+      const T* u2 = j >= 2 ? bk + (j - 2) * nS2 : (j == 1 ? i1 : i2);
+      launch_multi<T>(H, nblk, rpb, rpw, Gg, n, S2, u1, u2, u, L, falpha, R.F.as<T>(), ptot,
+                      static_cast<int32_t>(j), 0, nullptr, IR);
+      ++launches;
+      if (H.nI > 0 && H.has_xinv) {
+        launch_iface<T>(H, S2, wI, u);
+        ++launches;
+      }
+    }
+    last1 = steps >= 1 ? bk + (steps - 1) * nS2 : i1;
+    last2 = steps >= 2 ? bk + (steps - 2) * nS2 : (steps == 1 ? i1 : i2);
+    // gradient of segment k: its forward states (the last segment's from the
+    // forward run, the others recomputed in sweep k + 1) with the adjoint
+    // states of this sweep, stored in reverse time
+    const T* ck = rec.ckpt.as<T>() + k * 2 * nS;
+    const T* lam = bk + (La - 1) * nS2 + S;
+    if (k == nseg - 1)
+      launch_grad_multi<T>(H, G, S, rec.useg[0].as<T>(), nS, S, lam, -nS2, S2,
+                           static_cast<int32_t>(La), ck, ck + nS, dgK.as<double>(),
+                           dgM.as<double>());
+    else
+      launch_grad_multi<T>(H, G, S, buf[(k + 1) & 1].as<T>(), nS2, S2, lam, -nS2, S2,
+                           static_cast<int32_t>(La), ck, ck + nS, dgK.as<double>(),
+                           dgM.as<double>());
+    ++launches;
+  }
+  finish_timing(H, sampler, n_t - rec.t0, launches);
+  BFE_HIP(hipMemcpy(g_stiff, dgK.p, dgK.bytes, hipMemcpyDeviceToHost));
+  BFE_HIP(hipMemcpy(g_dens, dgM.p, dgM.bytes, hipMemcpyDeviceToHost));
 }
 
 }  // namespace
@@ -3402,6 +3712,29 @@ bfe_status bfe_run_checkpointed(bfe_handle h, int32_t slot, int32_t n_t, const i
   });
 }
 
+bfe_status bfe_run_checkpointed_batched(bfe_handle h, int32_t slot, int32_t n_t, int32_t n_src,
+                                        const int32_t* frc_idx, const int32_t* frc_src, int32_t p,
+                                        const double* frcs, int32_t frc_rows,
+                                        const int32_t* hist_idx, int32_t q, double* hist_out,
+                                        int32_t ckpt_every, int32_t* ti_start_out) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (ckpt_every < 1) throw std::invalid_argument("checkpoint interval must be >= 1");
+    if (slot < 0) throw std::invalid_argument("slot must be >= 0");
+    if (n_src < 1 || n_src > 32)
+      throw std::invalid_argument("a batched checkpointed run takes 1 to 32 sources");
+    auto rec = std::make_shared<AdjointRecord>();
+    h->adj_records.erase(slot);
+    if (h->prec == BFE_PREC_DOUBLE)
+      run_batched<double>(*h, n_t, n_src, frc_idx, frc_src, p, frcs, frc_rows, hist_idx, q,
+                          hist_out, ti_start_out, rec.get(), ckpt_every);
+    else
+      run_batched<float>(*h, n_t, n_src, frc_idx, frc_src, p, frcs, frc_rows, hist_idx, q,
+                         hist_out, ti_start_out, rec.get(), ckpt_every);
+    h->adj_records[slot] = rec;
+  });
+}
+
 bfe_status bfe_set_gradient_elements(bfe_handle h, int32_t n_el, int32_t kmax,
                                      const int32_t* el_dofs, const double* el_K,
                                      const double* el_m, const double* el_mu) {
@@ -3454,10 +3787,14 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad, doub
     if (!G) throw std::invalid_argument("gradient elements not set");
     if (!g_stiffness || !g_density) throw std::invalid_argument("NULL gradient output");
     if (rec->q > 0 && hist_grad == nullptr) throw std::invalid_argument("NULL history gradient");
-    if (h->prec == BFE_PREC_DOUBLE)
-      run_adjoint<double>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
-    else
-      run_adjoint<float>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+    const bool batched = rec->batched;
+    if (h->prec == BFE_PREC_DOUBLE) {
+      if (batched) run_adjoint_batched<double>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+      else run_adjoint<double>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+    } else {
+      if (batched) run_adjoint_batched<float>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+      else run_adjoint<float>(*h, *rec, *G, hist_grad, g_stiffness, g_density);
+    }
     h->adj_records.erase(slot);  // the forward states were consumed
   });
 }

@@ -219,12 +219,30 @@ bfe_status bfe_run_checkpointed(bfe_handle h, int32_t slot, int32_t n_t, const i
                                 int32_t ckpt_every, int32_t* ti_start_out);
 
 /*
+ * bfe_run_checkpointed_batched — bfe_run_batched (arguments as there, at
+ * most 32 sources) that also keeps, under record `slot`, the two states
+ * (n x n_src each) at the start of every ckpt_every-step segment and all
+ * states of the last segment, for one following bfe_adjoint, which then
+ * returns the gradient of J summed over the sources. It marches the CSR
+ * operator (the one the adjoint's recomputation uses).
+ */
+bfe_status bfe_run_checkpointed_batched(bfe_handle h, int32_t slot, int32_t n_t, int32_t n_src,
+                                        const int32_t* frc_idx, const int32_t* frc_src, int32_t p,
+                                        const double* frcs, int32_t frc_rows,
+                                        const int32_t* hist_idx, int32_t q, double* hist_out,
+                                        int32_t ckpt_every, int32_t* ti_start_out);
+
+/*
  * bfe_adjoint — gradients of a scalar J(hist) of the checkpointed run held
  * under `slot` (which it consumes).
- *   hist_grad (q x n_t)  dJ/dhist, the layout of hist_out
- *   g_stiffness, g_density (n_el)  dJ/ds_e and dJ/drho_e
+ *   hist_grad  dJ/dhist, the layout of that run's hist_out: q x n_t
+ *              (bfe_run_checkpointed), q x n_src x n_t
+ *              (bfe_run_checkpointed_batched)
+ *   g_stiffness, g_density (n_el)  dJ/ds_e and dJ/drho_e (summed over the
+ *              sources of a batched run)
  * The reverse sweep recomputes each segment from its checkpoint in the same
- * launches that march the adjoint over the following segment.
+ * launches that march the adjoint over the following segment (for a batched
+ * run: one multi-source march of 2 n_src columns per step).
  */
 bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
                        double* g_stiffness, double* g_density);
