@@ -326,6 +326,42 @@ def fn_get_edges(els):
     return uniq, els_with_ed
 
 
+def fn_find_free_edges(els):
+    """Edges with exactly one element on them and two nodes
+    (code/fn_find_free_edges.m:8-10), sorted as fn_get_edges sorts them."""
+    ed, ewe = fn_get_edges(els)
+    j = ((ewe > 0).sum(axis=1) == 1) & ((ed > 0).sum(axis=1) == 2)
+    return ed[j]
+
+
+def fn_add_scatterer(mod, matls, scat_pts, scat_matl):
+    """code/fn_add_scatterer.m:1-37: removes the interface elements, then sets
+    the elements whose centres lie inside scat_pts to material scat_matl, or,
+    for scat_matl = 0, removes them and the nodes left unused (a subdomain
+    model's bdry_lyrs and main_nd_i follow its nodes), and adds the
+    fluid-solid interface elements again; int_el_i flags the elements inside
+    the subdomain's inner boundary."""
+    mod = dict(mod)
+    keep = np.asarray(mod["el_typ_i"], dtype=object) != "ASI2D2"
+    for k in ("els", "el_mat_i", "el_abs_i", "el_typ_i"):
+        mod[k] = np.asarray(mod[k])[keep]
+    if scat_matl > 0:
+        mod.pop("el_typ_i")
+        mod = fn_set_els_inside_bdry_to_mat(mod, scat_pts, scat_matl)
+    else:
+        keep = ~fn_elements_in_region(mod, scat_pts)
+        for k in ("els", "el_mat_i", "el_abs_i", "el_typ_i"):
+            mod[k] = np.asarray(mod[k])[keep]
+        mod["nds"], mod["els"], old_nds, _ = fn_remove_unused_nodes(mod["nds"], mod["els"])
+        for k in ("bdry_lyrs", "main_nd_i"):
+            if k in mod:
+                mod[k] = np.asarray(mod[k])[old_nds - 1]
+    mod = fn_add_fluid_solid_interface_els(mod, matls)
+    if "inner_bndry_pts" in mod:
+        mod["int_el_i"] = fn_elements_in_region(mod, mod["inner_bndry_pts"])
+    return mod
+
+
 def fn_add_fluid_solid_interface_els(mod, matls, options=None):
     opts = {"interface_el_name": "ASI2D2", "fluid_el_names": ["AC2D3"], "solid_el_names": ["CPE3"]}
     opts.update(options or {})

@@ -13,6 +13,16 @@
                                                   pristine FMC and the
                                                   displacements at the subdomain
                                                   boundaries
+  fn_create_subdomain(mn_mod, matls, inner_bdry, abs_layer_thick)
+                                                  subdoms/fn_create_subdomain.m:1-87
+  fn_run_subdomain_model(main, fe_options)        subdoms/fn_run_subdomain_model.m:1-116:
+                                                  every transmitter's incident
+                                                  field injected at each
+                                                  subdomain's boundary layers, all
+                                                  transmitters marched together,
+                                                  and the scattered field added
+                                                  to the FMC through the
+                                                  receivers' impulse responses
 
 The convolution and the force conversion run on the GPU (bfe_convolve,
 bfe_disps_to_forces); the main model's transmitters are marched together by
@@ -29,7 +39,9 @@ import scipy.sparse as sp
 
 from . import _lib
 from .assembly import find_dof_in_use_and_max_dof_per_el, fn_build_global_matrices_v5
-from .mesh import fn_gaussian_pulse
+from .mesh import (fn_add_fluid_solid_interface_els, fn_calc_element_centres,
+                   fn_dist_point_to_bdry_2D, fn_elements_in_region, fn_find_free_edges,
+                   fn_gaussian_pulse, fn_remove_unused_nodes)
 from .options import set_default_fields
 from .solver import make_plan
 
@@ -221,3 +233,213 @@ def fn_run_main_model(main, fe_options=None):
     res["fmc"] = fn_parse_to_fmc(tmpl, steps_valid, fe_res, main["trans"], main["inp"]["sig"], o)
     main["res"] = res
     return main
+
+
+def _adjacent_els(els, consider, choose):
+    """(adjacent, common nodes) of subdoms/fn_create_subdomain.m:93-108: the
+    elements of `choose` that share a node with an element of `consider`, and
+    those shared nodes as MATLAB's els(tmp) lists them (column-major order,
+    repeats kept)."""
+    un = np.unique(els[consider])
+    un = un[un != 0]
+    tmp = np.isin(els, un) & choose[:, None]
+    common = els.ravel(order="F")[tmp.ravel(order="F")]
+    return tmp.any(axis=1), common
+
+
+def fn_create_subdomain(mn_mod, matls, inner_bdry, abs_layer_thick, abs_layer_start_bdry=None):
+    """Subdomain model of the main model's elements inside inner_bdry
+    (subdoms/fn_create_subdomain.m:1-87): four rings of elements around them
+    give the boundary layers 1-4 of their nodes (bdry_lyrs), the nodes of
+    layer 3 start the absorbing region (unless abs_layer_start_bdry is
+    given), whose elements carry el_abs_i = distance / abs_layer_thick and
+    end at 1; the interface elements are made again on the new mesh.
+    Returns the model with nds, els, el_mat_i, el_abs_i, el_typ_i,
+    bdry_lyrs, main_nd_i (1-based main node of each node), main_int_el_i,
+    inner_bndry_pts, outer_bndry_pts and int_el_i."""
+    nds = np.asarray(mn_mod["nds"], dtype=np.float64)
+    els = np.asarray(mn_mod["els"])
+    mat = np.asarray(mn_mod["el_mat_i"])
+    if "el_typ_i" in mn_mod:
+        typ = np.asarray(mn_mod["el_typ_i"], dtype=object)
+    else:
+        typ = np.asarray([matls[i - 1]["el_typ"] for i in mat], dtype=object)
+    bdry_lyrs = np.zeros(nds.shape[0], dtype=np.int64)
+    el_abs = np.zeros(els.shape[0])
+    inner_bdry = np.asarray(inner_bdry, dtype=np.float64)
+    el_used = np.asarray(fn_elements_in_region({"nds": nds, "els": els}, inner_bdry), dtype=bool)
+    main_int_el_i = np.nonzero(el_used)[0] + 1
+    start = None
+    for i in range(1, 5):
+        el_i, bdry_nds = _adjacent_els(els, el_used, ~el_used)
+        bdry_lyrs[bdry_nds - 1] = i
+        el_used = el_used | el_i
+        if i == 3:  # the third layer starts the absorbing region
+            start = (nds[bdry_nds - 1] if abs_layer_start_bdry is None
+                     else np.asarray(abs_layer_start_bdry, dtype=np.float64))
+    # the main model's interface elements go; they are made again below
+    keep = typ != "ASI2D2"
+    els, mat, el_abs, typ, el_used = els[keep], mat[keep], el_abs[keep], typ[keep], el_used[keep]
+    cand = ~el_used
+    if cand.any():
+        el_abs[cand] = fn_dist_point_to_bdry_2D(fn_calc_element_centres(nds, els[cand]),
+                                                start) / abs_layer_thick
+    keep = el_abs <= 1
+    els, mat, el_abs, typ = els[keep], mat[keep], el_abs[keep], typ[keep]
+    nds2, els2, old_nds, _ = fn_remove_unused_nodes(nds, els)
+    dm = {"nds": nds2, "els": els2, "el_mat_i": mat, "el_abs_i": el_abs, "el_typ_i": typ,
+          "bdry_lyrs": bdry_lyrs[old_nds - 1], "main_nd_i": old_nds,
+          "main_int_el_i": main_int_el_i, "inner_bndry_pts": inner_bdry}
+    dm = fn_add_fluid_solid_interface_els(dm, matls)
+    free_ed = fn_find_free_edges(dm["els"]).ravel(order="F")
+    dm["outer_bndry_pts"] = np.stack([dm["nds"][free_ed - 1, 0], dm["nds"][free_ed - 1, 1]], axis=1)
+    dm["int_el_i"] = fn_elements_in_region(dm, inner_bdry)
+    return dm
+
+
+def _main_subdomain_mappings(dom_mod, main_res):
+    """subdoms/fn_run_subdomain_model.m:119-145: the main model's monitored
+    entries on the subdomain's boundary nodes (mn_res_i, 0-based, in the
+    main monitor order), their global DOFs in the main model (gl_i, 1-based),
+    and the subdomain node, DOF and layer of each."""
+    lyrs = np.asarray(dom_mod["bdry_lyrs"])
+    bdry_nds = np.nonzero(lyrs > 0)[0] + 1
+    nb_m = np.asarray(dom_mod["main_nd_i"])[bdry_nds - 1]
+    mon_nds = np.asarray(main_res["mon_nds"])
+    mn_res_i = np.nonzero(np.isin(mon_nds, nb_m))[0]
+    gl_i = np.asarray(main_res["dsp_gi"])[mn_res_i]
+    Nb_m = mon_nds[mn_res_i]
+    # interp1(nb_m, bdry_nds, Nb_m, 'nearest'): every Nb_m is one of nb_m
+    sub_of = dict(zip(nb_m.tolist(), bdry_nds.tolist()))
+    b_nds = np.asarray([sub_of[int(x)] for x in Nb_m], dtype=np.int64)
+    return mn_res_i, gl_i, b_nds, np.asarray(main_res["mon_dfs"])[mn_res_i], lyrs[b_nds - 1]
+
+
+def fn_run_subdomain_model(main, fe_options=None):
+    """subdoms/fn_run_subdomain_model.m:1-116 after fn_run_main_model. For
+    each subdomain: the main model's boundary impulse responses of every
+    transmitter are convolved with the input signal and turned into the
+    'in' forces on boundary layers 2 and 3 (fn_convert_disps_to_forces_v2 on
+    the main model's matrices of the boundary DOFs); the subdomain model is
+    marched for all transmitters at once (bfe_run_batched: one source column
+    per transmitter, one signal row per force DOF, 64 columns per pass)
+    where the reference runs one FE step per transmitter; the marched
+    boundary displacements give the 'out' forces, which, convolved with each
+    receiver's impulse response at those DOFs and summed, are added to the
+    pristine FMC: main['doms'][d]['res']['fmc']. With finite
+    field_output_every_n_frames the transmitters run one by one through
+    fn_FE_entry_point, keeping each one's field frames."""
+    from .driver import fn_FE_entry_point, fn_nds_and_dfs_to_gi
+    n_tr = len(main["trans"])
+    o = set_default_fields(dict(fe_options or {}), {
+        "doms_to_run": [], "tx_trans": [], "rx_trans": [],
+        "field_output_every_n_frames": np.inf, "use_gpu_if_available": 1, "dof_to_use": [],
+        "solver_mode": "vel at curent time step"})
+    typs = sorted(set(np.asarray(main["mod"]["el_typ_i"], dtype=object).tolist()))
+    o["dof_to_use"] = find_dof_in_use_and_max_dof_per_el(typs, o["dof_to_use"])[0].tolist()
+    tx = list(o["tx_trans"]) or list(range(1, n_tr + 1))
+    rx = list(o["rx_trans"]) or list(range(1, n_tr + 1))
+    doms = list(o["doms_to_run"]) or list(range(1, len(main["doms"]) + 1))
+    time = np.asarray(main["inp"]["time"], dtype=np.float64)
+    sig = np.asarray(main["inp"]["sig"], dtype=np.float64).ravel()
+    n_t = time.size
+    dt = time[1] - time[0]
+    res_m = main["res"]
+    K, C, M = res_m["mats"]["K"], res_m["mats"]["C"], res_m["mats"]["M"]
+    mn_all = np.stack([np.asarray(res_m["mon_nds"]), np.asarray(res_m["mon_dfs"])], axis=1)
+    field = np.isfinite(o["field_output_every_n_frames"])
+    main = dict(main)
+    main["doms"] = [dict(d) for d in main["doms"]]
+    for d in doms:
+        dom = main["doms"][d - 1]
+        dm = dom["mod"]
+        mn_res_i, gl_i, b_nds, b_dfs, b_lyrs = _main_subdomain_mappings(dm, res_m)
+        g0 = gl_i - 1
+        K_sub, C_sub, M_sub = K[g0][:, g0], C[g0][:, g0], M[g0][:, g0]
+        # incident boundary displacements of every transmitter, convolved with
+        # the input signal in one batched convolution (fn_run_subdomain_model.m:46)
+        inc = np.concatenate([np.asarray(res_m["trans"][t - 1]["dsps"])[mn_res_i] for t in tx])
+        inc = fn_convolve(inc, sig, 2).reshape(len(tx), mn_res_i.size, n_t)
+        steps, frce_set = [], None
+        for si, t in enumerate(tx):
+            frcs, frce_set, _, _ = fn_convert_disps_to_forces_v2(K_sub, C_sub, M_sub, dt, inc[si],
+                                                                 b_lyrs, "in", o["solver_mode"])
+            steps.append({"load": {"frc_nds": b_nds[frce_set], "frc_dfs": b_dfs[frce_set],
+                                   "frcs": frcs, "time": time},
+                          "mon": {"nds": b_nds, "dfs": b_dfs,
+                                  "field_output_every_n_frames": o["field_output_every_n_frames"]}})
+        if field:
+            res = fn_FE_entry_point(dm, main["matls"], steps, o)
+            dsps = [r["dsps"] for r in res]
+        else:
+            dsps = _march_steps_batched(dm, main["matls"], steps, o, time)
+        out = {"fmc": {k: (np.array(v, copy=True) if isinstance(v, np.ndarray) else v)
+                       for k, v in res_m["fmc"].items()}}
+        if field:
+            out["trans"] = [dict() for _ in range(n_tr)]
+            for si, t in enumerate(tx):
+                out["trans"][t - 1] = {"fld": res[si]["fld"], "fld_time": res[si]["fld_time"]}
+        fmc = out["fmc"]
+        mn_nds = np.asarray(dm["main_nd_i"])[b_nds[frce_set] - 1]
+        sel = _rows_in(mn_all, np.stack([mn_nds, b_dfs[frce_set]], axis=1))
+        for si, t in enumerate(tx):
+            frcs, _, _, _ = fn_convert_disps_to_forces_v2(K_sub, C_sub, M_sub, dt, dsps[si], b_lyrs,
+                                                          "out", o["solver_mode"])
+            # every receiver's impulse responses at the force DOFs, convolved
+            # row by row with the forces and summed (:92-102)
+            R = np.concatenate([np.asarray(res_m["trans"][r - 1]["dsps"])[sel] for r in rx])
+            G = np.tile(frcs, (len(rx), 1))
+            scat = fn_convolve(R, G, 2).reshape(len(rx), frcs.shape[0], n_t).sum(axis=1)
+            for ri, r in enumerate(rx):
+                k = np.nonzero((np.asarray(fmc["tx"]) == t) & (np.asarray(fmc["rx"]) == r))[0]
+                fmc["time_data"][:, k] += scat[ri][:, None]
+        dom["res"] = out
+    return main
+
+
+def _rows_in(a, b):
+    """ismember(a, b, 'rows') for integer pairs."""
+    kb = {tuple(x) for x in np.asarray(b).tolist()}
+    return np.asarray([tuple(x) in kb for x in np.asarray(a).tolist()], dtype=bool)
+
+
+def _march_steps_batched(dom_mod, matls, steps, o, time):
+    """The load steps of one subdomain, all transmitters marched together:
+    the model assembled once, one source column per step, one signal row per
+    force DOF; returns each step's monitored displacements (rows in the
+    steps' monitor order, as fn_BristolFE_v2 returns res.dsps)."""
+    from .driver import fn_nds_and_dfs_to_gi
+    opts = set_default_fields(dict(o), {"damping_power_law": 3, "max_damping": 3.1415e7,
+                                        "max_stiffness_reduction": 0.01,
+                                        "solver_precision": "double"})
+    dm = dict(dom_mod)
+    if "el_abs_i" not in dm:
+        dm["el_abs_i"] = np.zeros(len(dm["el_mat_i"]))
+    K, C, M, gl = fn_build_global_matrices_v5(dm["nds"], dm["els"], dm["el_mat_i"], dm["el_abs_i"],
+                                              dm["el_typ_i"], matls, opts)
+    n_t = time.size
+    mon_gi, _, _, valid_m = fn_nds_and_dfs_to_gi(steps[0]["mon"]["nds"], steps[0]["mon"]["dfs"], gl)
+    fd, fs, rows = [], [], []
+    for s, st in enumerate(steps):
+        gi, _, _, valid = fn_nds_and_dfs_to_gi(st["load"]["frc_nds"], st["load"]["frc_dfs"], gl)
+        fr = np.atleast_2d(np.asarray(st["load"]["frcs"], dtype=np.float64))
+        fd.append(gi - 1)
+        fs.append(np.full(gi.size, s))
+        rows.append(fr[valid] if fr.shape[0] > 1 else np.repeat(fr, gi.size, axis=0))
+    fd, fs, F = np.concatenate(fd), np.concatenate(fs), np.concatenate(rows)
+    plan = make_plan(K, C, M, time[1] - time[0], opts["solver_mode"], opts["solver_precision"])
+    out = []
+    try:
+        nrow = int(np.nonzero(valid_m)[0].max()) + 1 if valid_m.any() else 0
+        S = len(steps)
+        for c0 in range(0, S, 64):
+            c1 = min(c0 + 64, S)
+            sel = (fs >= c0) & (fs < c1)
+            h, _ = plan.run_batched(n_t, c1 - c0, fd[sel], fs[sel] - c0, F[sel], mon_gi - 1)
+            for s in range(c1 - c0):
+                d = np.zeros((nrow, n_t))
+                d[np.nonzero(valid_m)[0]] = h[:, s, :]
+                out.append(d)
+    finally:
+        plan.close()
+    return out

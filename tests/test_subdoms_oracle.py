@@ -92,3 +92,51 @@ def test_parse_to_fmc_sums_or_weights_receiver_rows():
     np.testing.assert_array_equal(td[:, 1], 2 * rows[:4].sum(axis=0))
     np.testing.assert_array_equal(td[:, 2], 0.5 * rows[4] - rows[5])
     np.testing.assert_array_equal(td[:, 3], 2 * (0.5 * rows[4] - rows[5]))
+
+
+def _subdomain_case(scatterer):
+    m = PKG.mesh
+    mod, matls = solid_plate(6e-3, 3e-3, 0.1e-3)
+    inner = np.array([[0.5e-3, 0.5e-3], [5.5e-3, 0.5e-3], [5.5e-3, 5e-3], [0.5e-3, 5e-3]])
+    mod = m.fn_add_absorbing_layer(mod, inner, 0.5e-3)
+    box = np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]]) * 0.5e-3 + np.array([3e-3, 1.6e-3])
+    dm = PKG.subdoms.fn_create_subdomain(mod, matls, box, 0.5e-3)
+    if scatterer:
+        blob = m.fn_create_smooth_random_blob(0.5, 3, 64, np.random.default_rng(9)) * 0.3e-3
+        dm = m.fn_add_scatterer(dm, matls, blob + np.array([3e-3, 1.6e-3]), 0)
+    return mod, matls, box, dm
+
+
+def test_create_subdomain_structure():
+    """fn_create_subdomain.m:42-87 on a structured plate: the subdomain's
+    nodes are main-model nodes (main_nd_i), the inner elements keep no
+    absorbing index, each boundary layer i >= 2 lies next to layer i - 1, the
+    layers grow outward, the absorbing index stays in [0, 1], every inner
+    element is kept, and a void scatterer removes elements and nodes while
+    main_nd_i and bdry_lyrs follow the nodes."""
+    mod, matls, box, dm = _subdomain_case(False)
+    nds = np.asarray(mod["nds"])
+    np.testing.assert_array_equal(dm["nds"], nds[dm["main_nd_i"] - 1])
+    lyrs = dm["bdry_lyrs"]
+    counts = [int(np.count_nonzero(lyrs == i)) for i in range(1, 5)]
+    assert all(c > 0 for c in counts) and counts == sorted(counts)
+    els = dm["els"]
+    for i in range(2, 5):
+        # every layer-i node shares an element with a layer-(i-1) node
+        have = np.isin(els, np.nonzero(lyrs == i - 1)[0] + 1).any(axis=1)
+        near = np.unique(els[have])
+        assert np.all(np.isin(np.nonzero(lyrs == i)[0] + 1, near))
+    a = dm["el_abs_i"]
+    assert a.min() >= 0 and a.max() <= 1 and a.max() > 0.9
+    assert not a[dm["int_el_i"]].any()
+    # the main model's inner elements are all in the subdomain
+    n_inner = dm["main_int_el_i"].size
+    assert n_inner == int(np.count_nonzero(dm["int_el_i"])) > 0
+    # layer-1 nodes are nodes of the inner elements
+    inner_nodes = np.unique(els[dm["int_el_i"]])
+    assert np.all(np.isin(np.nonzero(lyrs == 1)[0] + 1, inner_nodes))
+    _, _, _, dv = _subdomain_case(True)
+    assert dv["els"].shape[0] < els.shape[0] and dv["nds"].shape[0] < dm["nds"].shape[0]
+    np.testing.assert_array_equal(dv["nds"], nds[dv["main_nd_i"] - 1])
+    kept = np.isin(dm["main_nd_i"], dv["main_nd_i"])
+    np.testing.assert_array_equal(dv["bdry_lyrs"], lyrs[kept])
