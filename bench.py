@@ -369,7 +369,7 @@ def bench_march(name, reps, warmup, operator="auto", timing_every=16, cpu=True, 
     return out
 
 
-def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
+def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None, n_t=None):
     """Batched multi-source workload: 64 FMC transmitters on the plate plus 8
     subdomain models in one 64-column march (bfe_run_batched), against 64
     single-source launches (bfe_run) with the best single-source operator."""
@@ -378,7 +378,7 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None):
     from bristolfe_v2_amd.solver import make_plan
     from oracle import v6 as ov6
     t0 = time.perf_counter()
-    c = configs.fmc_multimodel()
+    c = configs.fmc_multimodel(**({} if n_t is None else {"n_t": n_t}))
     K, C, M = c["K"], c["C"], c["M"]
     n, S, n_t = K.shape[0], c["n_src"], len(c["time"])
     fd, fs, hd, F = c["frc_dof"], c["frc_src"], c["hist_dof"], c["frcs"]
@@ -681,8 +681,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 2 GiB device copy (counter calibration)")
     ap.add_argument("--fmc-shape", default=None,
-                    help="fmc_batched: fixed launch shape rows,rpw,remap,group,tile instead of the "
-                         "autotuner")
+                    help="fmc_batched: fixed launch shape rows,rpw,remap,group,tile[,npw] instead "
+                         "of the autotuner (npw > 0: the multi-source element product with npw "
+                         "nodes per wavefront)")
     ap.add_argument("--fd-elements", type=int, default=None)
     ap.add_argument("--fd-step", type=float, default=1e-5)
     args = ap.parse_args()
@@ -703,9 +704,13 @@ def main():
     if wl == "fmc_batched":
         shape = None
         if args.fmc_shape:
-            r_, w_, x_, g_, tile_ = (int(v) for v in args.fmc_shape.split(","))
-            shape = dict(rows=r_, rpw=w_, remap=x_, group=g_, tile=tile_)
-        body = bench_fmc(args.steps, args.warmup, autotune_full=True, shape=shape)
+            v_ = [int(v) for v in args.fmc_shape.split(",")]
+            r_, w_, x_, g_, tile_ = v_[:5]
+            npw_ = v_[5] if len(v_) > 5 else 0
+            shape = dict(rows=r_, rpw=w_, remap=x_, group=g_, tile=tile_,
+                         operator="ebe" if npw_ > 0 else "csr", npw=npw_)
+        body = bench_fmc(args.steps, args.warmup, autotune_full=True, shape=shape,
+                         n_t=args.time_steps)
         body["ms_per_step"] = round(body["batched_wall_s"] * 1e3, 3)
         print(json.dumps(line(body, args, "batched per-source speedup")), flush=True)
         return

@@ -32,6 +32,7 @@ TUNE_PERSISTENT_GRID = 10
 TUNE_MULTI_OPERATOR = 11
 TUNE_MULTI_NODES_PER_WAVE = 12
 TUNE_MULTI_SLIDE = 13
+TUNE_ADJOINT_PAIRS = 14
 
 # Every symbol the header declares.
 EXPORTS = (
@@ -57,7 +58,7 @@ class BfeInfo(C.Structure):
         ("ebe_slots", C.c_int32), ("ebe_templates", C.c_int32), ("ebe_pattern", C.c_int32),
         ("ebe_elements", C.c_int64), ("ebe_check", C.c_double), ("ebe_classes", C.c_int32),
         ("ebe_groups", C.c_int32), ("multi_group", C.c_int32), ("ebe_sparse", C.c_int32),
-        ("multi_operator", C.c_int32),
+        ("multi_operator", C.c_int32), ("grad_pairs", C.c_int32),
     ]
 
 

@@ -1046,27 +1046,140 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // forcing (f_src = lane), interface and monitor handling are those of
 // k_step_multi on the node's rows; the lists hold the group's local rows
 // v*DPN + c and are grouped per workgroup of this launch.
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false>
-__global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
-    int64_t n_nodes, int64_t nblk, int32_t npw, int32_t S, int32_t slide, int64_t n,
-    const int32_t* __restrict__ dof0, ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an,
-    const uint32_t* __restrict__ pmask, const uint32_t* __restrict__ smask,
-    const int32_t* __restrict__ nbr, const int32_t* __restrict__ adj,
-    const T* __restrict__ el_scale, const T* __restrict__ anchor, const T* __restrict__ srows,
-    const T* __restrict__ stencil, const T* __restrict__ alpha, const T* __restrict__ beta,
-    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
-    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
-    T* __restrict__ hist_prev, IfaceRows<T> I, int32_t coef_per_node) {
+
+// K u1 on node v's DPN rows for every lane (source): the slot product, or
+// the fixed stencil where every element is present with scale 1. X(j, d) is
+// the u1 value of neighbour class j, component d.
+template <typename T, typename PT, int DPN, bool ANCHOR, bool SP, typename XF>
+__device__ __forceinline__ void ebe_multi_sum(
+    int64_t v, int64_t n_nodes, uint32_t pm, uint32_t sm, const AnchorSlots<PT::NS>& an,
+    const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
+    const T* __restrict__ anchor, const T* __restrict__ srows, const T* __restrict__ stencil,
+    XF X, T (&acc)[DPN]) {
   constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN;
   constexpr uint32_t kAll = (1u << NS) - 1u;
-  const int64_t lb = xcd_block(blockIdx.x, nblk);
-  const int64_t wave =
-      lb * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+  if (!ANCHOR && (pm & kAll) == kAll && sm == 0u) {
+    // every element present with scale 1: the fixed stencil sum_j B_j u_j
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+#pragma unroll
+      for (int d = 0; d < DPN; ++d)
+#pragma unroll
+        for (int c = 0; c < DPN; ++c) acc[c] += stencil[(j * DPN + c) * DPN + d] * X(j, d);
+    return;
+  }
+  const bool irr = (pm >> 31) != 0u;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (((pm >> k) & 1u) == 0u) continue;  // empty slot (uniform branch)
+    T s = T(1);
+    if ((sm >> k) & 1u) {
+      if (ANCHOR && !irr) {
+        const int64_t a = v + an.base[k];
+        s = (a < 0 || a >= an.len) ? T(1) : anchor[a];
+      } else {
+        s = el_scale[adj[static_cast<int64_t>(k) * n_nodes + v]];
+      }
+    }
+    const T* R = srows + k * (DPN * KE);
+    T y[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) y[c] = T(0);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b)
+#pragma unroll
+      for (int d = 0; d < DPN; ++d)
+#pragma unroll
+        for (int c = 0; c < DPN; ++c)
+          if (!SP || PT::nzbit(k, c, b * DPN + d))
+            y[c] += R[c * KE + b * DPN + d] * X(PT::cls[k][b], d);
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) acc[c] += s * y[c];
+  }
+}
+
+// The update of node v's rows r0 .. r0 + DPN - 1 for one lane (k_step_multi's
+// per row): a = u1 and b = u2 of the rows, acc = (K u1) of the rows.
+template <typename T, bool DAMPED, int DPN>
+__device__ __forceinline__ void ebe_multi_update(
+    int64_t lb, int64_t v, int64_t r0, int lane, int32_t S, uint32_t pm, const T (&acc)[DPN],
+    const T (&a)[DPN], const T (&b)[DPN], const T* __restrict__ alpha,
+    const T* __restrict__ beta, int32_t coef_per_node, const T* __restrict__ u1,
+    const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t,
+    int32_t q, T* __restrict__ hist_prev, const IfaceRows<T>& I) {
+  const uint32_t undamped = (pm >> 30) & 1u;
+  const int64_t l0 = v * DPN;
+  T un[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) {
+    const int64_t ci = coef_per_node ? v : r0 + c;
+    const T ka = alpha[ci] * acc[c];
+    if constexpr (DAMPED) {
+      const T be = undamped ? T(2) : beta[ci];
+      un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
+    } else {
+      un[c] = (T(2) * a[c] - b[c]) - ka;
+    }
+  }
+  for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.f_row[i];
+    if (rr / DPN == v && L.f_src[i] == lane) {
+      const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
+#pragma unroll
+      for (int c = 0; c < DPN; ++c)
+        if (rr == l0 + c) un[c] += f;
+    }
+  }
+  for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
+    const int64_t rr = L.i_row[i];
+    if (rr / DPN != v) continue;
+    const int32_t k = L.i_k[i];
+    T c1 = T(0), c2 = T(0);
+    for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
+      const int64_t cc = static_cast<int64_t>(I.ccol[p]) * S + lane;
+      c1 += I.cval[p] * u1[cc];
+      c2 += I.cval[p] * u2[cc];
+    }
+#pragma unroll
+    for (int c = 0; c < DPN; ++c)
+      if (rr == l0 + c) {
+        un[c] += I.gfix[k] * b[c] + I.sq * c1 + I.sr * c2;
+        if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un[c];
+      }
+  }
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) stream_store<true>(un[c], u + (r0 + c) * S + lane);
+  if (hist_prev != nullptr)
+    for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i) {
+      const int64_t rr = L.m_row[i];
+      if (rr / DPN == v) {
+#pragma unroll
+        for (int c = 0; c < DPN; ++c)
+          if (rr == l0 + c) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a[c];
+      }
+    }
+}
+
+// Nodes vbeg .. vend - 1 of one wavefront, one after the other (the sliding
+// neighbour rows along runs of regular nodes when slide is set).
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP>
+__device__ __forceinline__ void ebe_multi_nodes(
+    int64_t lb, int64_t vbeg, int64_t vend, int lane, int32_t S, int32_t slide, int64_t n_nodes,
+    int64_t n, const int32_t* __restrict__ dof0, const ClassOffsets<PT::NC>& co,
+    const AnchorSlots<PT::NS>& an, const uint32_t* __restrict__ pmask,
+    const uint32_t* __restrict__ smask, const int32_t* __restrict__ nbr,
+    const int32_t* __restrict__ adj, const T* __restrict__ el_scale,
+    const T* __restrict__ anchor, const T* __restrict__ srows, const T* __restrict__ stencil,
+    const T* __restrict__ alpha, const T* __restrict__ beta, const T* __restrict__ u1,
+    const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t,
+    int32_t q, T* __restrict__ hist_prev, const IfaceRows<T>& I, int32_t coef_per_node) {
+  constexpr int NS = PT::NS, NC = PT::NC;
+  constexpr uint32_t kAll = (1u << NS) - 1u;
   const bool active = lane < S;
-  const int64_t vbeg = wave * npw;
-  if (vbeg >= n_nodes) return;  // uniform across the wavefront
-  const int64_t vend = min(vbeg + npw, n_nodes);
   auto load = [&](int64_t r) -> T { return active ? u1[r * S + lane] : T(0); };
   // first DOF of neighbour j on a regular node (an offset out of range
   // belongs to an absent neighbour, used only by empty slots)
@@ -1118,103 +1231,43 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
     }
     rprev = r0;
     T acc[DPN];
-#pragma unroll
-    for (int c = 0; c < DPN; ++c) acc[c] = T(0);
-    if (!ANCHOR && (pm & kAll) == kAll && sm == 0u) {
-      // every element present with scale 1: the fixed stencil sum_j B_j u_j
-#pragma unroll
-      for (int j = 0; j < NC; ++j)
-#pragma unroll
-        for (int d = 0; d < DPN; ++d)
-#pragma unroll
-          for (int c = 0; c < DPN; ++c) acc[c] += stencil[(j * DPN + c) * DPN + d] * x[j][d];
-    } else {
-      const bool irr = (pm >> 31) != 0u;
-#pragma unroll
-      for (int k = 0; k < NS; ++k) {
-        if (((pm >> k) & 1u) == 0u) continue;  // empty slot (uniform branch)
-        T s = T(1);
-        if ((sm >> k) & 1u) {
-          if (ANCHOR && !irr) {
-            const int64_t a = v + an.base[k];
-            s = (a < 0 || a >= an.len) ? T(1) : anchor[a];
-          } else {
-            s = el_scale[adj[static_cast<int64_t>(k) * n_nodes + v]];
-          }
-        }
-        const T* R = srows + k * (DPN * KE);
-        T y[DPN];
-#pragma unroll
-        for (int c = 0; c < DPN; ++c) y[c] = T(0);
-#pragma unroll
-        for (int b = 0; b < NPE; ++b)
-#pragma unroll
-          for (int d = 0; d < DPN; ++d)
-#pragma unroll
-            for (int c = 0; c < DPN; ++c)
-              if (!SP || PT::nzbit(k, c, b * DPN + d))
-                y[c] += R[c * KE + b * DPN + d] * x[PT::cls[k][b]][d];
-#pragma unroll
-        for (int c = 0; c < DPN; ++c) acc[c] += s * y[c];
-      }
-    }
+    ebe_multi_sum<T, PT, DPN, ANCHOR, SP>(v, n_nodes, pm, sm, an, adj, el_scale, anchor, srows,
+                                          stencil, [&](int j, int d) { return x[j][d]; }, acc);
     if (!active) continue;
-    // the update of the node's rows (k_step_multi's, per lane)
-    const uint32_t undamped = (pm >> 30) & 1u;
-    const int64_t l0 = v * DPN;
-    T a[DPN], b[DPN], un[DPN];
+    T a[DPN], b[DPN];
 #pragma unroll
     for (int c = 0; c < DPN; ++c) {
-      const int64_t r = r0 + c;
-      const int64_t ci = coef_per_node ? v : r;
       a[c] = x[PT::SELF][c];
-      b[c] = stream_load<true>(u2 + r * S + lane);
-      const T ka = alpha[ci] * acc[c];
-      if constexpr (DAMPED) {
-        const T be = undamped ? T(2) : beta[ci];
-        un[c] = be * a[c] + (T(1) - be) * b[c] - ka;
-      } else {
-        un[c] = (T(2) * a[c] - b[c]) - ka;
-      }
+      b[c] = stream_load<true>(u2 + (r0 + c) * S + lane);
     }
-    for (int32_t i = L.fptr[lb], ie = L.fptr[lb + 1]; i < ie; ++i) {
-      const int64_t rr = L.f_row[i];
-      if (rr / DPN == v && L.f_src[i] == lane) {
-        const T f = f_alpha[i] * F[static_cast<int64_t>(t) * F_rows + L.f_frow[i]];
-#pragma unroll
-        for (int c = 0; c < DPN; ++c)
-          if (rr == l0 + c) un[c] += f;
-      }
-    }
-    for (int32_t i = L.iptr[lb], ie = L.iptr[lb + 1]; i < ie; ++i) {
-      const int64_t rr = L.i_row[i];
-      if (rr / DPN != v) continue;
-      const int32_t k = L.i_k[i];
-      T c1 = T(0), c2 = T(0);
-      for (int64_t p = I.cptr[k], pe = I.cptr[k + 1]; p < pe; ++p) {
-        const int64_t cc = static_cast<int64_t>(I.ccol[p]) * S + lane;
-        c1 += I.cval[p] * u1[cc];
-        c2 += I.cval[p] * u2[cc];
-      }
-#pragma unroll
-      for (int c = 0; c < DPN; ++c)
-        if (rr == l0 + c) {
-          un[c] += I.gfix[k] * b[c] + I.sq * c1 + I.sr * c2;
-          if (I.wI != nullptr) I.wI[static_cast<int64_t>(k) * S + lane] = un[c];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < DPN; ++c) stream_store<true>(un[c], u + (r0 + c) * S + lane);
-    if (hist_prev != nullptr)
-      for (int32_t i = L.mptr[lb], ie = L.mptr[lb + 1]; i < ie; ++i) {
-        const int64_t rr = L.m_row[i];
-        if (rr / DPN == v) {
-#pragma unroll
-          for (int c = 0; c < DPN; ++c)
-            if (rr == l0 + c) hist_prev[L.m_k[i] + static_cast<int64_t>(q) * lane] = a[c];
-        }
-      }
+    ebe_multi_update<T, DAMPED, DPN>(lb, v, r0, lane, S, pm, acc, a, b, alpha, beta,
+                                     coef_per_node, u1, u2, u, L, f_alpha, F, F_rows, t, q,
+                                     hist_prev, I);
   }
+}
+
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false>
+__global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
+    int64_t n_nodes, int64_t nblk, int32_t npw, int32_t S, int32_t slide, int64_t n,
+    const int32_t* __restrict__ dof0, ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an,
+    const uint32_t* __restrict__ pmask, const uint32_t* __restrict__ smask,
+    const int32_t* __restrict__ nbr, const int32_t* __restrict__ adj,
+    const T* __restrict__ el_scale, const T* __restrict__ anchor, const T* __restrict__ srows,
+    const T* __restrict__ stencil, const T* __restrict__ alpha, const T* __restrict__ beta,
+    const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
+    const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
+    T* __restrict__ hist_prev, IfaceRows<T> I, int32_t coef_per_node) {
+  const int64_t lb = xcd_block(blockIdx.x, nblk);
+  const int64_t wave =
+      lb * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const int64_t vbeg = wave * npw;
+  if (vbeg >= n_nodes) return;  // uniform across the wavefront
+  const int64_t vend = min(vbeg + npw, n_nodes);
+  ebe_multi_nodes<T, DAMPED, PT, DPN, ANCHOR, SP>(
+      lb, vbeg, vend, lane, S, slide, n_nodes, n, dof0, co, an, pmask, smask, nbr, adj, el_scale,
+      anchor, srows, stencil, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, I,
+      coef_per_node);
 }
 
 // Pattern id of a layout's class table, 0 if none matches.
@@ -1407,6 +1460,182 @@ __global__ __launch_bounds__(kBlock) void k_grad_multi(
 }
 
 // ---------------------------------------------------------------------------
+// Gradients of the batched adjoint as node-pair correlations. With element
+// vertices grouped into nodes of GD consecutive DOFs, the stiffness term of
+// element e is
+//   sum_t lam_t' K_e u_{t-1} = sum_{i,j in e} K_e[i, j] C[d_i, d_j],
+//   C[a, b] = sum_{t, s} lam_a(t, s) u_b(t - 1, s),
+// and the mass term needs D0[a] = sum lam_a(t) u_a(t), C[a, a] and
+// D2[a] = sum lam_a(t) u_a(t - 2). C is needed only on node pairs that share
+// an element, so k_pair_acc gives one thread per (node a, source s) that
+// walks the segment once, reading node a's and its EM upper neighbours' u and
+// lam rows (16 rows per step for 2D solids against the 24 that the two
+// elements per node read in k_grad_multi) and accumulating in FP64:
+//   self block C[a, a] (GD x GD), D0[a], D2[a] and, for each upper
+//   neighbour b (b > a, k-th in a's list), C[a, b] and C[b, a].
+// Sums over the sources are reduced across the node's GS lanes and added to
+// node a's record (pair_stride doubles) by its first lane; records persist
+// over the segments, and k_pair_contract forms every element's gradient from
+// them once after the sweep. Absent neighbours (slot k beyond a's list) read
+// node a's own rows and their sums are never used.
+template <int GD, int EM>
+constexpr int pair_stride() {
+  return GD * GD + 2 * GD + EM * 2 * GD * GD;
+}
+
+template <typename T, int GD, int EM>
+__global__ __launch_bounds__(kBlock) void k_pair_acc(
+    int64_t n_groups, int32_t S, int32_t GS, const int32_t* __restrict__ gdof,
+    const int32_t* __restrict__ gnb, const T* __restrict__ U, int64_t us, int64_t ud,
+    const T* __restrict__ Lam, int64_t ls, int64_t ld, int32_t Ls,
+    const T* __restrict__ um1, const T* __restrict__ um2, double* __restrict__ acc) {
+  constexpr int BS = GD * GD, STRIDE = pair_stride<GD, EM>();
+  const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = tid / GS;
+  const int s = static_cast<int>(tid - a * GS);
+  const bool on = a < n_groups && s < S;
+  const int64_t aa = a < n_groups ? a : 0;  // threads past the end read node 0, then drop it
+  const int ss = s < S ? s : 0;
+  const int64_t da = gdof[aa];
+  int64_t db[EM];
+#pragma unroll
+  for (int k = 0; k < EM; ++k) {
+    const int32_t b = gnb[static_cast<int64_t>(k) * n_groups + aa];
+    db[k] = b >= 0 ? static_cast<int64_t>(gdof[b]) : da;
+  }
+  double ua1[GD], ua2[GD], ub1[EM][GD];
+#pragma unroll
+  for (int c = 0; c < GD; ++c) {
+    ua1[c] = static_cast<double>(um1[(da + c) * S + ss]);
+    ua2[c] = static_cast<double>(um2[(da + c) * S + ss]);
+  }
+#pragma unroll
+  for (int k = 0; k < EM; ++k)
+#pragma unroll
+    for (int c = 0; c < GD; ++c) ub1[k][c] = static_cast<double>(um1[(db[k] + c) * S + ss]);
+  double cs[BS], d0[GD], d2[GD], cab[EM][BS], cba[EM][BS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i) cs[i] = 0.0;
+#pragma unroll
+  for (int c = 0; c < GD; ++c) d0[c] = d2[c] = 0.0;
+#pragma unroll
+  for (int k = 0; k < EM; ++k)
+#pragma unroll
+    for (int i = 0; i < BS; ++i) cab[k][i] = cba[k][i] = 0.0;
+  for (int32_t t = 0; t < Ls; ++t) {
+    double ua[GD], la[GD], ub[EM][GD], lb[EM][GD];
+#pragma unroll
+    for (int c = 0; c < GD; ++c) {
+      ua[c] = static_cast<double>(U[t * us + (da + c) * ud + ss]);
+      la[c] = static_cast<double>(Lam[t * ls + (da + c) * ld + ss]);
+    }
+#pragma unroll
+    for (int k = 0; k < EM; ++k)
+#pragma unroll
+      for (int c = 0; c < GD; ++c) {
+        ub[k][c] = static_cast<double>(U[t * us + (db[k] + c) * ud + ss]);
+        lb[k][c] = static_cast<double>(Lam[t * ls + (db[k] + c) * ld + ss]);
+      }
+#pragma unroll
+    for (int c = 0; c < GD; ++c) {
+      d0[c] += la[c] * ua[c];
+      d2[c] += la[c] * ua2[c];
+#pragma unroll
+      for (int d = 0; d < GD; ++d) cs[c * GD + d] += la[c] * ua1[d];
+    }
+#pragma unroll
+    for (int k = 0; k < EM; ++k)
+#pragma unroll
+      for (int c = 0; c < GD; ++c)
+#pragma unroll
+        for (int d = 0; d < GD; ++d) {
+          cab[k][c * GD + d] += la[c] * ub1[k][d];
+          cba[k][c * GD + d] += lb[k][c] * ua1[d];
+        }
+#pragma unroll
+    for (int c = 0; c < GD; ++c) {
+      ua2[c] = ua1[c];
+      ua1[c] = ua[c];
+    }
+#pragma unroll
+    for (int k = 0; k < EM; ++k)
+#pragma unroll
+      for (int c = 0; c < GD; ++c) ub1[k][c] = ub[k][c];
+  }
+  // the node's record: self block, D0, D2, then (C[a,b], C[b,a]) per slot
+  double r[STRIDE];
+#pragma unroll
+  for (int i = 0; i < BS; ++i) r[i] = cs[i];
+#pragma unroll
+  for (int c = 0; c < GD; ++c) {
+    r[BS + c] = d0[c];
+    r[BS + GD + c] = d2[c];
+  }
+#pragma unroll
+  for (int k = 0; k < EM; ++k)
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+      r[BS + 2 * GD + k * 2 * BS + i] = cab[k][i];
+      r[BS + 2 * GD + k * 2 * BS + BS + i] = cba[k][i];
+    }
+#pragma unroll
+  for (int i = 0; i < STRIDE; ++i) {
+    double x = on ? r[i] : 0.0;
+    for (int o = GS >> 1; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);  // the node's lanes
+    r[i] = x;
+  }
+  if (s == 0 && a < n_groups) {
+    double* out = acc + a * STRIDE;
+#pragma unroll
+    for (int i = 0; i < STRIDE; ++i) out[i] += r[i];
+  }
+}
+
+// Element gradients from the node-pair records: eoff[e, i, j] is the offset
+// of the GD x GD block C[vertex i's node, vertex j's node] in acc (-1 where a
+// vertex is absent), egrp[e, i] the node of vertex i.
+//   gK_e -= dt^2 sum_{i,j,c,d} K_e[i GD + c, j GD + d] C_ij[c, d]
+//   gM_e -= sum_{i,c} m_e[i GD + c] (w0 D0 + w1 C[a, a]_cc + w2 D2)[c]
+template <typename T, int KV, int GD>
+__global__ __launch_bounds__(kBlock) void k_pair_contract(
+    int32_t n_el, int32_t stride, const int32_t* __restrict__ egrp,
+    const int64_t* __restrict__ eoff, const T* __restrict__ el_K, const T* __restrict__ el_m,
+    const T* __restrict__ el_w, const double* __restrict__ acc, double dt2,
+    double* __restrict__ gK, double* __restrict__ gM) {
+  constexpr int KM = KV * GD, BS = GD * GD;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n_el) return;
+  double sk = 0.0, sm = 0.0;
+#pragma unroll
+  for (int i = 0; i < KV; ++i)
+#pragma unroll
+    for (int j = 0; j < KV; ++j) {
+      const int64_t off = eoff[(e * KV + i) * KV + j];
+      if (off < 0) continue;
+#pragma unroll
+      for (int c = 0; c < GD; ++c)
+#pragma unroll
+        for (int d = 0; d < GD; ++d)
+          sk += static_cast<double>(el_K[(e * KM + i * GD + c) * KM + j * GD + d]) *
+                acc[off + c * GD + d];
+    }
+  const double w0 = static_cast<double>(el_w[3 * e]), w1 = static_cast<double>(el_w[3 * e + 1]),
+               w2 = static_cast<double>(el_w[3 * e + 2]);
+#pragma unroll
+  for (int i = 0; i < KV; ++i) {
+    const int32_t a = egrp[e * KV + i];
+    if (a < 0) continue;
+    const double* r = acc + static_cast<int64_t>(a) * stride;
+#pragma unroll
+    for (int c = 0; c < GD; ++c)
+      sm += static_cast<double>(el_m[e * KM + i * GD + c]) *
+            (w0 * r[BS + c] + w1 * r[c * GD + c] + w2 * r[BS + GD + c]);
+  }
+  gK[e] -= dt2 * sk;
+  gM[e] -= sm;
+}
+
+// ---------------------------------------------------------------------------
 // Device buffers owned by a handle or a run.
 struct DevBuf {
   void* p = nullptr;
@@ -1589,6 +1818,7 @@ struct bfe_plan_s {
   int32_t multi_ebe = 0;             // batched march on the element operator when attached
   int32_t multi_npw = 4;             // nodes per wavefront of k_step_ebe_multi
   int32_t multi_pipeline = 1;        // k_step_ebe_multi: neighbour rows slide along node runs
+  int32_t grad_pairs = 1;            // batched adjoint: node-pair gradients where possible
   int32_t last_multi_op = BFE_OPERATOR_CSR;  // operator of the last batched run
   ~bfe_plan_s() {
     if (stream) {
@@ -2655,7 +2885,149 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
 struct GradElements {
   int32_t n_el = 0, kmax = 0;
   DevBuf dofs, K, m, w;
+  // node-pair form of the batched gradient (pair_gd = 0: not available)
+  int32_t pair_gd = 0, pair_em = 0, pair_kv = 0, pair_stride = 0;
+  int64_t n_groups = 0;
+  DevBuf gdof, gnb, egrp, eoff;  // node first DOF, upper neighbours, element nodes and blocks
 };
+
+// Node-pair accumulation over one segment (k_pair_acc) and the final
+// contraction (k_pair_contract), for the shapes that have kernels.
+bool pair_shape_ok(int gd, int em, int kv) {
+  const bool acc_ok = (gd == 1 && em >= 2 && em <= 6) || (gd == 2 && em >= 2 && em <= 4);
+  const bool con_ok = (gd == 1 && (kv == 2 || kv == 3 || kv == 4)) || (gd == 2 && (kv == 2 || kv == 3 || kv == 4));
+  return acc_ok && con_ok;
+}
+
+// Node-pair layout of gradient elements (see k_pair_acc): the vertices of
+// every element are GD consecutive DOFs starting at a node's first DOF (or
+// all absent), each DOF belongs to one node, and no node has more than
+// em_max neighbours above it. Returns false when the elements do not have
+// that form; the batched adjoint then uses k_grad_multi.
+bool pair_layout(const std::vector<int32_t>& d, int64_t ne, int32_t kmax, int64_t n, int gd,
+                 int em_max, GradElements& G) {
+  if (kmax % gd != 0) return false;
+  const int kv = kmax / gd;
+  std::vector<int32_t> start(n, -1);  // DOF -> first DOF of its node
+  std::vector<int8_t> comp(n, -1);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int v = 0; v < kv; ++v) {
+      const int32_t* x = &d[e * kmax + v * gd];
+      if (x[0] < 0) {
+        for (int c = 1; c < gd; ++c)
+          if (x[c] >= 0) return false;
+        continue;
+      }
+      for (int c = 0; c < gd; ++c) {
+        if (x[c] != x[0] + c) return false;
+        if (start[x[c]] < 0) {
+          start[x[c]] = x[0];
+          comp[x[c]] = static_cast<int8_t>(c);
+        } else if (start[x[c]] != x[0] || comp[x[c]] != c) {
+          return false;
+        }
+      }
+    }
+  std::vector<int32_t> gid(n, -1), gdof;
+  for (int64_t i = 0; i < n; ++i)
+    if (start[i] == i) {
+      gid[i] = static_cast<int32_t>(gdof.size());
+      gdof.push_back(static_cast<int32_t>(i));
+    }
+  const int64_t ng = static_cast<int64_t>(gdof.size());
+  if (ng == 0) return false;
+  std::vector<int32_t> egrp(static_cast<size_t>(ne) * kv, -1);
+  std::vector<uint64_t> pairs;
+  for (int64_t e = 0; e < ne; ++e) {
+    for (int v = 0; v < kv; ++v) {
+      const int32_t x = d[e * kmax + v * gd];
+      egrp[e * kv + v] = x >= 0 ? gid[x] : -1;
+    }
+    for (int i = 0; i < kv; ++i)
+      for (int j = i + 1; j < kv; ++j) {
+        const int32_t A = egrp[e * kv + i], B = egrp[e * kv + j];
+        if (A < 0 || B < 0 || A == B) continue;
+        const uint64_t lo = static_cast<uint64_t>(std::min(A, B)), hi = static_cast<uint64_t>(std::max(A, B));
+        pairs.push_back((lo << 32) | hi);
+      }
+  }
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  std::vector<int32_t> deg(ng, 0);
+  for (uint64_t p : pairs) ++deg[p >> 32];
+  int em = 1;
+  for (int32_t x : deg) em = std::max(em, static_cast<int>(x));
+  if (em > em_max) return false;
+  em = std::max(em, 2);
+  if (!pair_shape_ok(gd, em, kv)) return false;
+  // upper neighbours, ascending, slot-major; slot of each pair
+  std::vector<int32_t> gnb(static_cast<size_t>(em) * ng, -1), fill(ng, 0);
+  for (uint64_t p : pairs) {
+    const int32_t A = static_cast<int32_t>(p >> 32), B = static_cast<int32_t>(p & 0xffffffffu);
+    gnb[static_cast<int64_t>(fill[A]++) * ng + A] = B;
+  }
+  auto slot_of = [&](int32_t A, int32_t B) {
+    for (int k = 0; k < em; ++k)
+      if (gnb[static_cast<int64_t>(k) * ng + A] == B) return k;
+    return -1;
+  };
+  const int bs = gd * gd;
+  const int stride = bs + 2 * gd + em * 2 * bs;
+  std::vector<int64_t> eoff(static_cast<size_t>(ne) * kv * kv, -1);
+  for (int64_t e = 0; e < ne; ++e)
+    for (int i = 0; i < kv; ++i)
+      for (int j = 0; j < kv; ++j) {
+        const int32_t A = egrp[e * kv + i], B = egrp[e * kv + j];
+        if (A < 0 || B < 0) continue;
+        int64_t off;
+        if (A == B) off = static_cast<int64_t>(A) * stride;
+        else if (A < B) off = static_cast<int64_t>(A) * stride + bs + 2 * gd + slot_of(A, B) * 2 * bs;
+        else off = static_cast<int64_t>(B) * stride + bs + 2 * gd + slot_of(B, A) * 2 * bs + bs;
+        eoff[(e * kv + i) * kv + j] = off;
+      }
+  G.pair_gd = gd;
+  G.pair_em = em;
+  G.pair_kv = kv;
+  G.pair_stride = stride;
+  G.n_groups = ng;
+  G.gdof = upload(gdof);
+  G.gnb = upload(gnb);
+  G.egrp = upload(egrp);
+  G.eoff = upload(eoff);
+  return true;
+}
+
+template <typename T>
+void launch_pair_acc(const bfe_plan_s& H, const GradElements& G, int32_t S, const T* U,
+                     int64_t us, int64_t ud, const T* Lam, int64_t ls, int64_t ld, int32_t Ls,
+                     const T* um1, const T* um2, double* acc) {
+  int32_t GS = 1;
+  while (GS < S) GS <<= 1;
+  const int64_t threads = G.n_groups * GS;
+  const dim3 grid((threads + kBlock - 1) / kBlock), blk(kBlock);
+  const int32_t *gd = G.gdof.as<int32_t>(), *nb = G.gnb.as<int32_t>();
+#define BFE_PAIR(GD_, EM_)                                                                     case GD_ * 16 + EM_:                                                                           hipLaunchKernelGGL((k_pair_acc<T, GD_, EM_>), grid, blk, 0, H.stream, G.n_groups, S, GS,                        gd, nb, U, us, ud, Lam, ls, ld, Ls, um1, um2, acc);                       break;
+  switch (G.pair_gd * 16 + G.pair_em) {
+    BFE_PAIR(1, 2) BFE_PAIR(1, 3) BFE_PAIR(1, 4) BFE_PAIR(1, 5) BFE_PAIR(1, 6)
+    BFE_PAIR(2, 2) BFE_PAIR(2, 3) BFE_PAIR(2, 4)
+    default: throw std::invalid_argument("no node-pair kernel for this element layout");
+  }
+#undef BFE_PAIR
+}
+
+template <typename T>
+void launch_pair_contract(const bfe_plan_s& H, const GradElements& G, const double* acc,
+                          double* gK, double* gM) {
+  const dim3 grid((G.n_el + kBlock - 1) / kBlock), blk(kBlock);
+  const double dt2 = H.dt * H.dt;
+  const T *Kp = G.K.as<T>(), *mp = G.m.as<T>(), *wp = G.w.as<T>();
+#define BFE_PCON(KV_, GD_)                                                                       case KV_ * 16 + GD_:                                                                             hipLaunchKernelGGL((k_pair_contract<T, KV_, GD_>), grid, blk, 0, H.stream, G.n_el,                                G.pair_stride, G.egrp.as<int32_t>(), G.eoff.as<int64_t>(), Kp, mp, wp,                        acc, dt2, gK, gM);                                                          break;
+  switch (G.pair_kv * 16 + G.pair_gd) {
+    BFE_PCON(2, 1) BFE_PCON(3, 1) BFE_PCON(4, 1) BFE_PCON(2, 2) BFE_PCON(3, 2) BFE_PCON(4, 2)
+    default: throw std::invalid_argument("no node-pair contraction for this element layout");
+  }
+#undef BFE_PCON
+}
 
 template <typename T>
 void launch_grad(const bfe_plan_s& H, const GradElements& G, const T* U, const T* Lam, int32_t Ls,
@@ -3244,6 +3616,9 @@ void run_adjoint_batched(bfe_plan_s& H, AdjointRecord& rec, const GradElements& 
   DevBuf dgK(sizeof(double) * G.n_el), dgM(sizeof(double) * G.n_el);
   BFE_HIP(hipMemsetAsync(dgK.p, 0, dgK.bytes, st));
   BFE_HIP(hipMemsetAsync(dgM.p, 0, dgM.bytes, st));
+  const bool pairs = H.grad_pairs && G.pair_gd > 0;
+  DevBuf pacc(pairs ? sizeof(double) * static_cast<size_t>(G.n_groups) * G.pair_stride : 0);
+  if (pairs) BFE_HIP(hipMemsetAsync(pacc.p, 0, pacc.bytes, st));
   int64_t launches = 0;
   std::vector<T> Fj(static_cast<size_t>(ptot) * C);
   KernelSampler sampler(H, 0, 0);
@@ -3313,14 +3688,18 @@ void run_adjoint_batched(bfe_plan_s& H, AdjointRecord& rec, const GradElements& 
     // states of this sweep, stored in reverse time
     const T* ck = rec.ckpt.as<T>() + k * 2 * nS;
     const T* lam = bk + (La - 1) * nS2 + S;
-    if (k == nseg - 1)
-      launch_grad_multi<T>(H, G, S, rec.useg[0].as<T>(), nS, S, lam, -nS2, S2,
-                           static_cast<int32_t>(La), ck, ck + nS, dgK.as<double>(),
-                           dgM.as<double>());
+    const T* Useg = k == nseg - 1 ? rec.useg[0].as<T>() : buf[(k + 1) & 1].as<T>();
+    const int64_t us = k == nseg - 1 ? nS : nS2, ud = k == nseg - 1 ? S : S2;
+    if (pairs)
+      launch_pair_acc<T>(H, G, S, Useg, us, ud, lam, -nS2, S2, static_cast<int32_t>(La), ck,
+                         ck + nS, pacc.as<double>());
     else
-      launch_grad_multi<T>(H, G, S, buf[(k + 1) & 1].as<T>(), nS2, S2, lam, -nS2, S2,
-                           static_cast<int32_t>(La), ck, ck + nS, dgK.as<double>(),
-                           dgM.as<double>());
+      launch_grad_multi<T>(H, G, S, Useg, us, ud, lam, -nS2, S2, static_cast<int32_t>(La), ck,
+                           ck + nS, dgK.as<double>(), dgM.as<double>());
+    ++launches;
+  }
+  if (pairs) {
+    launch_pair_contract<T>(H, G, pacc.as<double>(), dgK.as<double>(), dgM.as<double>());
     ++launches;
   }
   finish_timing(H, sampler, n_t - rec.t0, launches);
@@ -3827,6 +4206,9 @@ bfe_status bfe_set_gradient_elements(bfe_handle h, int32_t n_el, int32_t kmax,
     std::vector<int32_t> d(el_dofs, el_dofs + ne * kmax);
     std::vector<double> K(el_K, el_K + ne * kmax * kmax), m(el_m, el_m + ne * kmax);
     G->dofs = upload(d);
+    // node-pair form for the batched adjoint: nodes of 2 DOFs, else of 1
+    if (!pair_layout(d, static_cast<int64_t>(ne), kmax, h->n, 2, 4, *G))
+      (void)pair_layout(d, static_cast<int64_t>(ne), kmax, h->n, 1, 6, *G);
     if (h->prec == BFE_PREC_DOUBLE) {
       G->K = upload(K); G->m = upload(m); G->w = upload(w);
     } else {
@@ -3948,6 +4330,9 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
       case BFE_TUNE_MULTI_SLIDE:
         h->multi_pipeline = value ? 1 : 0;
         break;
+      case BFE_TUNE_ADJOINT_PAIRS:
+        h->grad_pairs = value ? 1 : 0;
+        break;
       case BFE_TUNE_PERSISTENT_GRID:
         if (value < 0) throw std::invalid_argument("persistent grid cap must be >= 0");
         h->persistent_grid = value;
@@ -4007,6 +4392,7 @@ bfe_status bfe_info_get(bfe_handle h, bfe_info* info) {
     }
     I.multi_group = h->last_group;
     I.multi_operator = h->last_multi_op;
+    I.grad_pairs = h->grad_elems ? std::static_pointer_cast<GradElements>(h->grad_elems)->pair_gd : 0;
     *info = I;
   });
 }

@@ -80,6 +80,8 @@ typedef struct bfe_info {
   int32_t ebe_sparse;    /* element groups on a structurally sparse product
                             (zero template entries skipped; equal to rounding) */
   int32_t multi_operator; /* operator of the last batched run: BFE_OPERATOR_* */
+  int32_t grad_pairs;    /* DOFs per node of the node-pair form of the gradient
+                            elements (1 or 2; BFE_TUNE_ADJOINT_PAIRS), 0: none */
 } bfe_info;
 
 /* Timing of the last bfe_run / bfe_run_batched, measured with HIP events
@@ -306,6 +308,13 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * and gathers only the others; 0: every node gathers all its neighbour rows.
  * Results are identical. */
 #define BFE_TUNE_MULTI_SLIDE 13
+/* 1 (default): the batched adjoint (bfe_run_checkpointed_batched +
+ * bfe_adjoint) accumulates node-pair correlations of the adjoint and forward
+ * states over the sweep and forms the element gradients from them once, when
+ * the gradient elements' vertices are nodes of 1 or 2 consecutive DOFs with
+ * few neighbours (2D triangles); 0: per-element sums every segment
+ * (k_grad_multi). Results agree to rounding. */
+#define BFE_TUNE_ADJOINT_PAIRS 14
 /* operator of the single-source march once an element operator is attached
  * (default: the element operator) */
 #define BFE_TUNE_OPERATOR 3

@@ -192,7 +192,7 @@ def test_batched_element_product(name, mode):
         assert plan.info()["multi_operator"] == 1
         assert np.linalg.norm(hc) > 0
         assert rel_l2(he, hc) < 1e-13
-        for npw, slide in ((1, 1), (3, 1), (64, 1), (4, 0)):
+        for npw, slide in ((1, 1), (2, 1), (3, 1), (64, 1), (4, 0)):
             plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, npw)
             plan.set_tuning(_lib.TUNE_MULTI_SLIDE, slide)
             h2, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
