@@ -414,7 +414,8 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None, n_t=None):
     # both multi-source operators at the chosen shape, one full march each
     plan = res["double"]["plan"]
     ops = {}
-    for name, sel in (("csr", 0), ("element_product", 1)):
+    for name, sel in (("csr", _lib.OPERATOR_CSR), ("element_product", _lib.OPERATOR_EBE),
+                      ("node_blocks", _lib.OPERATOR_NODE)):
         plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, sel)
         if sel:
             plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, res["double"]["best"].get("npw") or 4)
@@ -457,10 +458,17 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None, n_t=None):
     info = plan.info()
     speedup = plain_wall / res["double"]["wall"]
     steps = res["double"]["steps"]
-    if res["double"]["best"].get("operator") == "ebe":
+    chosen = res["double"]["best"].get("operator")
+    if chosen == "ebe":
         # per DOF-step-source: the element data and coefficients once per step
         # for all S sources, u_n / u_{n-1} read and u_{n+1} written per source
         bpd = (ebe.ebe_bytes_per_dof(op, n, info["damped"]) - 24.0) / S + 24.0
+    elif chosen == "node":
+        # per node its 7 neighbour-class blocks (dpn x dpn), mask, alpha and
+        # beta once per step for all S sources; u_n / u_{n-1} / u_{n+1} per source
+        per_node = sum(g.n_nodes * (7 * g.dpn * g.dpn * 8 + 4 + (16 if info["damped"] else 8))
+                       for g in op.groups)
+        bpd = per_node / n / S + 24.0
     else:
         bpd = (12.0 * info["nnz_k"] / n + 4.0) / S + 24.0 + (16.0 / S if info["damped"] else 0.0)
     gb = bpd * n * S * steps / (res["double"]["loop_ms"] / 1e3) / 1e9
@@ -477,7 +485,10 @@ def bench_fmc(reps=2, warmup=1, autotune_full=False, shape=None, n_t=None):
                      "traffic_uncertainty": unc,
                      "kernel": (f"k_step_ebe_multi<double,true,PatTriRect,2,true> ({S} sources, "
                                 f"{res['double']['best']['npw']} nodes per wavefront)"
-                                if res["double"]["best"].get("operator") == "ebe" else
+                                if chosen == "ebe" else
+                                f"k_step_ebe_multi<double,true,PatTriRect,2,false,false,true> "
+                                f"(node-block form, {S} sources, {res['double']['best']['npw']} "
+                                f"nodes per wavefront)" if chosen == "node" else
                                 f"k_step_multi<double> ({S} sources, {res['double']['best']['rpw']} "
                                 f"rows per wavefront, {res['double']['best']['group']} rows per "
                                 f"gather)"),

@@ -22,6 +22,7 @@ TUNE_XCD_REMAP = 2
 TUNE_OPERATOR = 3
 OPERATOR_CSR = 0
 OPERATOR_EBE = 1
+OPERATOR_NODE = 2
 TUNE_EBE_GROUP = 4
 TUNE_EBE_KERNEL = 5
 TUNE_MULTI_ROWS_PER_WAVE = 6

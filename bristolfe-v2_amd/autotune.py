@@ -13,13 +13,15 @@ Tunables (none changes results):
           per byte of u)
   operator  the CSR operator, or (with an element operator attached whose
           groups all have a mesh-pattern kernel) the multi-source element
-          product with npw nodes per wavefront: 2, 4, 8
+          product with npw nodes per wavefront: 2, 4, 8, or (triangle
+          patterns) the same kernel on the node-block form of the element
+          operator
 Each candidate marches a short probe (a prefix of the real time axis) on the
 real operator; the fastest by HIP-event loop time wins. The search is staged:
 rows x rpw first (XCD order on, automatic group, widest tile), then each
 other group with one or two groups per wavefront over all rows, then the XCD
 order, then the tile width, each at the best shape found so far, then the
-element product's nodes per wavefront.
+element product's and the node-block form's nodes per wavefront.
 """
 from __future__ import annotations
 
@@ -62,7 +64,7 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
                  groups=(1, 2, 4, 8), repeats=2, ebe_npw=(2, 4, 8)):
     """Returns (best, table): best = dict(rows, rpw, remap, group, tile,
     operator, npw, ms_per_step); group is the one the library used, operator
-    "csr" or "ebe" (npw its nodes per wavefront)."""
+    "csr", "ebe" or "node" (npw its nodes per wavefront)."""
     frcs = np.atleast_2d(frcs)
     # the probe must include forced steps: start it at the first non-zero sample
     nz = np.nonzero(np.abs(frcs).sum(axis=0))[0]
@@ -70,23 +72,24 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
     probe = frcs[:, :t0 + probe_steps]
     table = []
 
-    def measure(r, w, x, g, tile, npw=0):
+    def measure(r, w, x, g, tile, npw=0, op=_lib.OPERATOR_EBE):
         plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_BLOCK, r)
         plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, w)
         plan.set_tuning(_lib.TUNE_XCD_REMAP, x)
         plan.set_tuning(_lib.TUNE_MULTI_GROUP, g)
         if npw:
             plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, npw)
-        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, 1 if npw else 0)
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, op if npw else _lib.OPERATOR_CSR)
         best = np.inf
         for _ in range(repeats):
             _, ms = run_tiled(plan, probe.shape[1], n_src, frc_dof, frc_src, probe, hist_dof, tile)
             best = min(best, ms)
         steps = max(int(plan.stats()["steps"]), 1)
         info = plan.info()
-        ebe = bool(npw) and info.get("multi_operator", 0) == 1
+        used = info.get("multi_operator", 0) if npw else _lib.OPERATOR_CSR
+        name = {_lib.OPERATOR_EBE: "ebe", _lib.OPERATOR_NODE: "node"}.get(used, "csr")
         rec = dict(rows=r, rpw=w, remap=x, group=int(info["multi_group"]), tile=tile,
-                   operator="ebe" if ebe else "csr", npw=int(npw) if ebe else 0,
+                   operator=name, npw=int(npw) if name != "csr" else 0,
                    ms_per_step=best / steps)
         table.append(rec)
         return rec
@@ -111,11 +114,14 @@ def tune_batched(plan, n_src, frc_dof, frc_src, frcs, hist_dof, probe_steps=120,
         best = keep(measure(best["rows"], best["rpw"], best["remap"], best["group"], tile), best)
     info = plan.info()
     if info.get("operator_kind", 0) == 1 and info.get("ebe_pattern", 0) != 0:
-        for npw in ebe_npw:  # the element product, where the plan can run it
-            rec = measure(best["rows"], best["rpw"], best["remap"], best["group"], best["tile"], npw)
-            if rec["operator"] != "ebe":
-                break
-            best = keep(rec, best)
+        # the element product and its node-block form, where the plan can run them
+        for op, name in ((_lib.OPERATOR_EBE, "ebe"), (_lib.OPERATOR_NODE, "node")):
+            for npw in ebe_npw:
+                rec = measure(best["rows"], best["rpw"], best["remap"], best["group"],
+                              best["tile"], npw, op)
+                if rec["operator"] != name:
+                    break
+                best = keep(rec, best)
     apply(plan, best)
     return best, table
 
@@ -126,7 +132,8 @@ def apply(plan, shape):
     plan.set_tuning(_lib.TUNE_MULTI_ROWS_PER_WAVE, shape["rpw"])
     plan.set_tuning(_lib.TUNE_XCD_REMAP, shape["remap"])
     plan.set_tuning(_lib.TUNE_MULTI_GROUP, shape["group"])
-    ebe = shape.get("operator", "csr") == "ebe"
-    plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, 1 if ebe else 0)
-    if ebe:
+    op = {"ebe": _lib.OPERATOR_EBE, "node": _lib.OPERATOR_NODE}.get(shape.get("operator", "csr"),
+                                                                    _lib.OPERATOR_CSR)
+    plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, op)
+    if op != _lib.OPERATOR_CSR:
         plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, shape["npw"])

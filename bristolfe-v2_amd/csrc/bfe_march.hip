@@ -1071,18 +1071,33 @@ __device__ __forceinline__ void ebe_multi_sum(
     return;
   }
   const bool irr = (pm >> 31) != 0u;
+  // every slot's scale read is issued before any product (wave-uniform
+  // scalar loads, branch-free: a slot that needs none reads entry 0 and
+  // keeps 1), so the node waits for them once instead of once per slot
+  T sc[NS];
+  if (ANCHOR && !irr) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const int64_t a = v + an.base[k];
+      const bool use = ((sm >> k) & 1u) && a >= 0 && a < an.len;
+      const T x = anchor[use ? a : 0];
+      sc[k] = use ? x : T(1);
+    }
+  } else {
+    int32_t el[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+      el[k] = ((sm >> k) & 1u) ? adj[static_cast<int64_t>(k) * n_nodes + v] : 0;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      const T x = el_scale[el[k]];
+      sc[k] = ((sm >> k) & 1u) ? x : T(1);
+    }
+  }
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     if (((pm >> k) & 1u) == 0u) continue;  // empty slot (uniform branch)
-    T s = T(1);
-    if ((sm >> k) & 1u) {
-      if (ANCHOR && !irr) {
-        const int64_t a = v + an.base[k];
-        s = (a < 0 || a >= an.len) ? T(1) : anchor[a];
-      } else {
-        s = el_scale[adj[static_cast<int64_t>(k) * n_nodes + v]];
-      }
-    }
+    const T s = sc[k];
     const T* R = srows + k * (DPN * KE);
     T y[DPN];
 #pragma unroll
@@ -1163,9 +1178,25 @@ __device__ __forceinline__ void ebe_multi_update(
     }
 }
 
+// K u1 on node v's DPN rows from the node's neighbour-class blocks
+// (node-block form): nbv holds the NC blocks of DPN x DPN, class-major.
+template <typename T, typename PT, int DPN, typename XF>
+__device__ __forceinline__ void node_block_sum(const T* __restrict__ nbv, XF X, T (&acc)[DPN]) {
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) acc[c] = T(0);
+#pragma unroll
+  for (int j = 0; j < PT::NC; ++j)
+#pragma unroll
+    for (int c = 0; c < DPN; ++c)
+#pragma unroll
+      for (int d = 0; d < DPN; ++d) acc[c] += nbv[(j * DPN + c) * DPN + d] * X(j, d);
+}
+
 // Nodes vbeg .. vend - 1 of one wavefront, one after the other (the sliding
-// neighbour rows along runs of regular nodes when slide is set).
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP>
+// neighbour rows along runs of regular nodes when slide is set). NB: K u1
+// from the node-block form nb (n_nodes x NC x DPN x DPN) instead of the slot
+// product.
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP, bool NB = false>
 __device__ __forceinline__ void ebe_multi_nodes(
     int64_t lb, int64_t vbeg, int64_t vend, int lane, int32_t S, int32_t slide, int64_t n_nodes,
     int64_t n, const int32_t* __restrict__ dof0, const ClassOffsets<PT::NC>& co,
@@ -1176,7 +1207,8 @@ __device__ __forceinline__ void ebe_multi_nodes(
     const T* __restrict__ alpha, const T* __restrict__ beta, const T* __restrict__ u1,
     const T* __restrict__ u2, T* __restrict__ u, const BlockLists& L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t,
-    int32_t q, T* __restrict__ hist_prev, const IfaceRows<T>& I, int32_t coef_per_node) {
+    int32_t q, T* __restrict__ hist_prev, const IfaceRows<T>& I, int32_t coef_per_node,
+    const T* __restrict__ nb) {
   constexpr int NS = PT::NS, NC = PT::NC;
   constexpr uint32_t kAll = (1u << NS) - 1u;
   const bool active = lane < S;
@@ -1194,6 +1226,10 @@ __device__ __forceinline__ void ebe_multi_nodes(
     const uint32_t pm = pmask[v];
     const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
     const int64_t r0 = dof0 != nullptr ? static_cast<int64_t>(dof0[v]) : v * DPN;
+    // u2 of the node's rows, issued with the neighbour gathers (one wait per node)
+    T b[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) b[c] = active ? stream_load<true>(u2 + (r0 + c) * S + lane) : T(0);
     if (pm >> 31) {  // irregular node: neighbours from the class-major table
 #pragma unroll
       for (int j = 0; j < NC; ++j) {
@@ -1231,22 +1267,25 @@ __device__ __forceinline__ void ebe_multi_nodes(
     }
     rprev = r0;
     T acc[DPN];
-    ebe_multi_sum<T, PT, DPN, ANCHOR, SP>(v, n_nodes, pm, sm, an, adj, el_scale, anchor, srows,
-                                          stencil, [&](int j, int d) { return x[j][d]; }, acc);
+    if constexpr (NB)
+      node_block_sum<T, PT, DPN>(nb + v * (NC * DPN * DPN), [&](int j, int d) { return x[j][d]; },
+                                 acc);
+    else
+      ebe_multi_sum<T, PT, DPN, ANCHOR, SP>(v, n_nodes, pm, sm, an, adj, el_scale, anchor,
+                                            srows, stencil,
+                                            [&](int j, int d) { return x[j][d]; }, acc);
     if (!active) continue;
-    T a[DPN], b[DPN];
+    T a[DPN];
 #pragma unroll
-    for (int c = 0; c < DPN; ++c) {
-      a[c] = x[PT::SELF][c];
-      b[c] = stream_load<true>(u2 + (r0 + c) * S + lane);
-    }
+    for (int c = 0; c < DPN; ++c) a[c] = x[PT::SELF][c];
     ebe_multi_update<T, DAMPED, DPN>(lb, v, r0, lane, S, pm, acc, a, b, alpha, beta,
                                      coef_per_node, u1, u2, u, L, f_alpha, F, F_rows, t, q,
                                      hist_prev, I);
   }
 }
 
-template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false>
+template <typename T, bool DAMPED, typename PT, int DPN, bool ANCHOR, bool SP = false,
+          bool NB = false>
 __global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
     int64_t n_nodes, int64_t nblk, int32_t npw, int32_t S, int32_t slide, int64_t n,
     const int32_t* __restrict__ dof0, ClassOffsets<PT::NC> co, AnchorSlots<PT::NS> an,
@@ -1256,7 +1295,7 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
     const T* __restrict__ stencil, const T* __restrict__ alpha, const T* __restrict__ beta,
     const T* __restrict__ u1, const T* __restrict__ u2, T* __restrict__ u, BlockLists L,
     const T* __restrict__ f_alpha, const T* __restrict__ F, int32_t F_rows, int32_t t, int32_t q,
-    T* __restrict__ hist_prev, IfaceRows<T> I, int32_t coef_per_node) {
+    T* __restrict__ hist_prev, IfaceRows<T> I, int32_t coef_per_node, const T* __restrict__ nb) {
   const int64_t lb = xcd_block(blockIdx.x, nblk);
   const int64_t wave =
       lb * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
@@ -1264,10 +1303,56 @@ __global__ __launch_bounds__(kBlock) void k_step_ebe_multi(
   const int64_t vbeg = wave * npw;
   if (vbeg >= n_nodes) return;  // uniform across the wavefront
   const int64_t vend = min(vbeg + npw, n_nodes);
-  ebe_multi_nodes<T, DAMPED, PT, DPN, ANCHOR, SP>(
+  ebe_multi_nodes<T, DAMPED, PT, DPN, ANCHOR, SP, NB>(
       lb, vbeg, vend, lane, S, slide, n_nodes, n, dof0, co, an, pmask, smask, nbr, adj, el_scale,
       anchor, srows, stencil, alpha, beta, u1, u2, u, L, f_alpha, F, F_rows, t, q, hist_prev, I,
-      coef_per_node);
+      coef_per_node, nb);
+}
+
+// Node-block form of one element group: block (j, c, d) of node v is the sum
+// over v's occupied slots k and vertices b of class j of s_k R_k[c, b DPN + d]
+// (the slot product of ebe_multi_sum with the u1 values left out), so
+// K u1 on node v's rows is sum_j B_j(v) u1(class j). One thread per node.
+template <typename T, typename PT, int DPN, bool ANCHOR>
+__global__ __launch_bounds__(kBlock) void k_node_blocks(
+    int64_t n_nodes, AnchorSlots<PT::NS> an, const uint32_t* __restrict__ pmask,
+    const uint32_t* __restrict__ smask, const int32_t* __restrict__ adj,
+    const T* __restrict__ el_scale, const T* __restrict__ anchor, const T* __restrict__ srows,
+    T* __restrict__ nb) {
+  constexpr int NPE = PT::NPE, NS = PT::NS, NC = PT::NC, KE = NPE * DPN, BS = NC * DPN * DPN;
+  constexpr uint32_t kAll = (1u << NS) - 1u;
+  const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (v >= n_nodes) return;
+  const uint32_t pm = pmask[v];
+  const uint32_t sm = ANCHOR ? (pm & kAll) : smask[v];
+  const bool irr = (pm >> 31) != 0u;
+  T B[BS];
+#pragma unroll
+  for (int i = 0; i < BS; ++i) B[i] = T(0);
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (((pm >> k) & 1u) == 0u) continue;
+    T s = T(1);
+    if ((sm >> k) & 1u) {
+      if (ANCHOR && !irr) {
+        const int64_t a = v + an.base[k];
+        s = (a < 0 || a >= an.len) ? T(1) : anchor[a];
+      } else {
+        s = el_scale[adj[static_cast<int64_t>(k) * n_nodes + v]];
+      }
+    }
+    const T* R = srows + k * (DPN * KE);
+#pragma unroll
+    for (int b = 0; b < NPE; ++b) {
+      const int j = PT::cls[k][b];
+#pragma unroll
+      for (int c = 0; c < DPN; ++c)
+#pragma unroll
+        for (int d = 0; d < DPN; ++d) B[(j * DPN + c) * DPN + d] += s * R[c * KE + b * DPN + d];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < BS; ++i) nb[v * BS + i] = B[i];
 }
 
 // Pattern id of a layout's class table, 0 if none matches.
@@ -1720,10 +1805,11 @@ struct EbeGroup {
   DevBuf dof0, adj, nodes, scale, srows, nbr, srows_c, pmask, smask, stencil, anchor;
   DevBuf alpha_n, beta_n;    // per node (coef_node)
   DevBuf iptr, irow, ik;     // interface rows (group-local rows) per workgroup
+  DevBuf nblk;               // node-block form (built at the first batched run that uses it)
   int64_t bytes() const {
     int64_t s = 0;
     for (const DevBuf* b : {&dof0, &adj, &nodes, &scale, &srows, &nbr, &srows_c, &pmask, &smask,
-                            &stencil, &anchor, &alpha_n, &beta_n, &iptr, &irow, &ik})
+                            &stencil, &anchor, &alpha_n, &beta_n, &iptr, &irow, &ik, &nblk})
       s += static_cast<int64_t>(b->bytes);
     return s;
   }
@@ -1815,7 +1901,7 @@ struct bfe_plan_s {
   int32_t use_graph = 0;             // single-source march captured in a hipGraph
   int32_t use_persistent = 0;        // single-source CSR march as one cooperative launch
   int32_t persistent_grid = 0;       // cap on its workgroups (0: as many as fit at once)
-  int32_t multi_ebe = 0;             // batched march on the element operator when attached
+  int32_t multi_ebe = 0;             // batched march's operator: BFE_OPERATOR_CSR, _EBE or _NODE
   int32_t multi_npw = 4;             // nodes per wavefront of k_step_ebe_multi
   int32_t multi_pipeline = 1;        // k_step_ebe_multi: neighbour rows slide along node runs
   int32_t grad_pairs = 1;            // batched adjoint: node-pair gradients where possible
@@ -3342,7 +3428,7 @@ template <typename T, bool D, typename PT, int DPN>
 void launch_ebe_multi_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr,
                           const EbeMultiLists& ML, int32_t npw, int32_t S, const T* u1,
                           const T* u2, T* u, const T* F, int32_t F_rows, int32_t t, int32_t q,
-                          T* hp, const IfaceRows<T>& IR) {
+                          T* hp, const IfaceRows<T>& IR, bool node_blocks) {
   ClassOffsets<PT::NC> co;
   for (int j = 0; j < PT::NC; ++j) co.off[j] = static_cast<int32_t>(Gr.doff[j]);
   AnchorSlots<PT::NS> an{};
@@ -3363,8 +3449,14 @@ void launch_ebe_multi_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& G
                        anchored ? Gr.anchor.as<T>() : static_cast<const T*>(nullptr),
                        Gr.srows_c.as<T>(), Gr.stencil.as<T>(), E.alpha_of<T>(Gr),
                        D ? E.beta_of<T>(Gr) : nullptr, u1, u2, u, L, ML.falpha.as<T>(), F,
-                       F_rows, t, q, hp, IR, Gr.coef_node);
+                       F_rows, t, q, hp, IR, Gr.coef_node, Gr.nblk.as<T>());
   };
+  if constexpr (PT::NPE == 3) {
+    if (node_blocks) {
+      go(k_step_ebe_multi<T, D, PT, DPN, false, false, true>);
+      return;
+    }
+  }
   if constexpr (PT::NS <= kAnchorMaxSlots) {
     if (anchored) {
       go(k_step_ebe_multi<T, D, PT, DPN, true>);
@@ -3382,12 +3474,12 @@ template <typename T>
 void launch_step_ebe_multi(const bfe_plan_s& H, const EbeOp& E,
                            const std::vector<EbeMultiLists>& ML, int32_t npw, int32_t S,
                            const T* u1, const T* u2, T* u, const T* F, int32_t F_rows, int32_t t,
-                           int32_t q, T* hp, const IfaceRows<T>& IR) {
+                           int32_t q, T* hp, const IfaceRows<T>& IR, bool node_blocks) {
   for (size_t g = 0; g < E.groups.size(); ++g) {
     const EbeGroup& Gr = E.groups[g];
 #define BFE_EM(D_, PT_, DPN_)                                                                  \
   launch_ebe_multi_pat<T, D_, PT_, DPN_>(H, E, Gr, ML[g], npw, S, u1, u2, u, F, F_rows, t, q, hp, \
-                                         IR)
+                                         IR, node_blocks)
 #define BFE_EM_D(PT_, DPN_) \
   if (H.damped) BFE_EM(true, PT_, DPN_); else BFE_EM(false, PT_, DPN_);
     switch (Gr.pattern * 10 + Gr.dpn) {
@@ -3402,6 +3494,54 @@ void launch_step_ebe_multi(const bfe_plan_s& H, const EbeOp& E,
     }
 #undef BFE_EM_D
 #undef BFE_EM
+  }
+}
+
+// Whether every group of the element operator can take the node-block form
+// (triangle patterns).
+bool node_blocks_ok(const bfe_plan_s& H) {
+  if (!ebe_multi_ok(H)) return false;
+  for (const auto& g : H.ebe->groups)
+    if (g.pattern != PatTriRect::ID && g.pattern != PatTriIso::ID) return false;
+  return true;
+}
+
+// Builds the node-block form of every group that does not hold it yet.
+template <typename T, typename PT, int DPN>
+void build_node_blocks_pat(const bfe_plan_s& H, EbeGroup& Gr) {
+  AnchorSlots<PT::NS> an{};
+  const bool anchored = PT::NS <= kAnchorMaxSlots && Gr.anchor.bytes > 0;
+  if (anchored) {
+    for (int k = 0; k < PT::NS; ++k)
+      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
+    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+  }
+  Gr.nblk = DevBuf(sizeof(T) * static_cast<size_t>(Gr.n_nodes) * PT::NC * DPN * DPN);
+  const dim3 grid((Gr.n_nodes + kBlock - 1) / kBlock), blk(kBlock);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, blk, 0, H.stream, Gr.n_nodes, an, Gr.pmask.as<uint32_t>(),
+                       Gr.smask.as<uint32_t>(), Gr.adj.as<int32_t>(), Gr.scale.as<T>(),
+                       anchored ? Gr.anchor.as<T>() : static_cast<const T*>(nullptr),
+                       Gr.srows_c.as<T>(), Gr.nblk.as<T>());
+  };
+  if (anchored) go(k_node_blocks<T, PT, DPN, true>);
+  else go(k_node_blocks<T, PT, DPN, false>);
+  BFE_HIP(hipGetLastError());
+}
+
+template <typename T>
+void ensure_node_blocks(const bfe_plan_s& H, EbeOp& E) {
+  for (auto& Gr : E.groups) {
+    if (Gr.nblk.bytes > 0) continue;
+    switch (Gr.pattern * 10 + Gr.dpn) {
+      case PatTriRect::ID * 10 + 1: build_node_blocks_pat<T, PatTriRect, 1>(H, Gr); break;
+      case PatTriRect::ID * 10 + 2: build_node_blocks_pat<T, PatTriRect, 2>(H, Gr); break;
+      case PatTriRect::ID * 10 + 3: build_node_blocks_pat<T, PatTriRect, 3>(H, Gr); break;
+      case PatTriIso::ID * 10 + 1: build_node_blocks_pat<T, PatTriIso, 1>(H, Gr); break;
+      case PatTriIso::ID * 10 + 2: build_node_blocks_pat<T, PatTriIso, 2>(H, Gr); break;
+      case PatTriIso::ID * 10 + 3: build_node_blocks_pat<T, PatTriIso, 3>(H, Gr); break;
+      default: throw std::invalid_argument("no node-block form for this element group");
+    }
   }
 }
 
@@ -3446,8 +3586,12 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
   // the element operator's multi-source product when every group has a
   // pattern kernel, else the row-grouped CSR operator (a checkpointed run
   // stays on the CSR operator that the adjoint's recomputation uses)
-  const EbeOp* E = (H.multi_ebe && rec == nullptr && ebe_multi_ok(H)) ? H.ebe.get() : nullptr;
-  H.last_multi_op = E ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
+  const bool nbf = H.multi_ebe == BFE_OPERATOR_NODE && rec == nullptr && node_blocks_ok(H);
+  const EbeOp* E = ((H.multi_ebe == BFE_OPERATOR_EBE && rec == nullptr && ebe_multi_ok(H)) || nbf)
+                       ? H.ebe.get()
+                       : nullptr;
+  H.last_multi_op = nbf ? BFE_OPERATOR_NODE : E ? BFE_OPERATOR_EBE : BFE_OPERATOR_CSR;
+  if (nbf) ensure_node_blocks<T>(H, *H.ebe);
   const int32_t G = E ? 0 : resolve_group<T>(H, rpw);
   H.last_group = G;
   const int64_t rows_per_wg = static_cast<int64_t>(rpb) * rpw;
@@ -3502,7 +3646,8 @@ void run_batched(bfe_plan_s& H, int32_t n_t, int32_t n_src, const int32_t* frc_i
     }
     sampler.before(t);
     if (E) {
-      launch_step_ebe_multi<T>(H, *E, ML, H.multi_npw, S, u1, u2, u, F, frc_rows, ti, q, hp, IR);
+      launch_step_ebe_multi<T>(H, *E, ML, H.multi_npw, S, u1, u2, u, F, frc_rows, ti, q, hp, IR,
+                               nbf);
       launches += static_cast<int64_t>(E->groups.size());
     } else {
       launch_multi<T>(H, nblk, rpb, rpw, G, n, S, u1, u2, u, L, falpha, F, frc_rows, ti, q, hp,
@@ -4318,9 +4463,10 @@ bfe_status bfe_set_tuning(bfe_handle h, int32_t key, int32_t value) {
         h->use_persistent = value ? 1 : 0;
         break;
       case BFE_TUNE_MULTI_OPERATOR:
-        if (value != BFE_OPERATOR_CSR && value != BFE_OPERATOR_EBE)
-          throw std::invalid_argument("operator must be BFE_OPERATOR_CSR or BFE_OPERATOR_EBE");
-        h->multi_ebe = value == BFE_OPERATOR_EBE ? 1 : 0;
+        if (value != BFE_OPERATOR_CSR && value != BFE_OPERATOR_EBE && value != BFE_OPERATOR_NODE)
+          throw std::invalid_argument(
+              "operator must be BFE_OPERATOR_CSR, BFE_OPERATOR_EBE or BFE_OPERATOR_NODE");
+        h->multi_ebe = value;
         break;
       case BFE_TUNE_MULTI_NODES_PER_WAVE:
         if (value < 1 || value > 1024)

@@ -295,10 +295,11 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
  * is attached: BFE_OPERATOR_EBE marches the multi-source element product
  * (k_step_ebe_multi: one wavefront per node, lane = source, each wavefront
  * marching a run of consecutive nodes) when every element group has a
- * mesh-pattern kernel, else the row-grouped CSR operator is used;
- * BFE_OPERATOR_CSR (the default: the faster of the two on the 2D FMC
- * workload) always uses the CSR operator. bfe_info multi_operator reports
- * the operator of the last batched run. Results agree to rounding. */
+ * mesh-pattern kernel, BFE_OPERATOR_NODE the same kernel on the node-block
+ * form (below) when every group is a triangle pattern, else the row-grouped
+ * CSR operator is used; BFE_OPERATOR_CSR (the default) always uses the CSR
+ * operator. bfe_info multi_operator reports the operator of the last
+ * batched run. Results agree to rounding. */
 #define BFE_TUNE_MULTI_OPERATOR 11
 /* consecutive nodes each wavefront of the multi-source element product
  * marches (1 to 1024; default 4) */
@@ -320,6 +321,13 @@ bfe_status bfe_adjoint(bfe_handle h, int32_t slot, const double* hist_grad,
 #define BFE_TUNE_OPERATOR 3
 #define BFE_OPERATOR_CSR 0
 #define BFE_OPERATOR_EBE 1
+/* batched march only: the node-block form of the element operator, per node
+ * the DOF blocks of K for each neighbour class (summed once from the element
+ * templates on the device, at the first batched run that asks for it); K u1
+ * is then sum_j B_j(v) u1(neighbour j), 28 products per 2D solid node against
+ * the slot product's 84. Triangle patterns only; other groups fall back to
+ * the CSR operator. */
+#define BFE_OPERATOR_NODE 2
 /* slots of the element operator whose loads are issued together in the
  * generic gather kernel (2, 4 or 8; default 8), and whether a mesh-pattern
  * kernel is used when the mesh matches one (0 or 1; default 1) */

@@ -122,8 +122,8 @@ def test_run_tiled_passes_each_pass_its_own_signal_rows():
 
 def test_element_product_stage_when_an_element_operator_is_attached():
     """With an element operator whose groups have pattern kernels, the last
-    stage times the multi-source element product at 2, 4 and 8 nodes per
-    wavefront, and keeps it when it is faster."""
+    stages time the multi-source element product and its node-block form at
+    2, 4 and 8 nodes per wavefront, and keep the fastest."""
 
     class P(SyntheticPlan):
         def info(self):
@@ -131,8 +131,10 @@ def test_element_product_stage_when_an_element_operator_is_attached():
                     "multi_operator": self.shape["op"]}
 
     def cost(sh):
-        if sh.get("op"):
+        if sh.get("op") == 1:
             return 0.5 + 0.01 * abs(sh["npw"] - 4)
+        if sh.get("op") == 2:
+            return 0.45 + 0.01 * abs(sh["npw"] - 2)
         return _cost(sh)
 
     class Q(P):
@@ -147,6 +149,7 @@ def test_element_product_stage_when_an_element_operator_is_attached():
     fd = np.arange(64)
     best, table = autotune.tune_batched(plan, 64, fd, np.arange(64), np.ones((1, 400)), fd,
                                         probe_steps=20, repeats=1)
-    assert best["operator"] == "ebe" and best["npw"] == 4
-    assert plan.shape["op"] == 1 and plan.shape["npw"] == 4
+    assert best["operator"] == "node" and best["npw"] == 2
+    assert plan.shape["op"] == 2 and plan.shape["npw"] == 2
     assert sorted(r["npw"] for r in table if r["operator"] == "ebe") == [2, 4, 8]
+    assert sorted(r["npw"] for r in table if r["operator"] == "node") == [2, 4, 8]

@@ -174,7 +174,8 @@ def test_batched_element_product(name, mode):
     (irregular nodes at the mesh seams), the coupled solid/water model (two
     groups, a node -> DOF table, interface rows) and a C3D4 cube (stencil and
     structurally sparse slot paths). Nodes per wavefront from 1 to 64, with
-    and without the sliding neighbour rows, give the same bits."""
+    and without the sliding neighbour rows, give the same bits. The node-block
+    form of the triangle groups matches CSR to 1e-13 as well."""
     print("case: batched element product", name, mode, flush=True)
     from bristolfe_v2_amd import _lib
     K, C, M, dt, op, src, hi = _ebe_case(name)
@@ -197,6 +198,19 @@ def test_batched_element_product(name, mode):
             plan.set_tuning(_lib.TUNE_MULTI_SLIDE, slide)
             h2, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
             np.testing.assert_array_equal(h2, he)
+        # the node-block form (triangle patterns; the tets fall back to CSR)
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, _lib.OPERATOR_NODE)
+        hn, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+        if name == "cube":
+            assert plan.info()["multi_operator"] == _lib.OPERATOR_CSR
+            np.testing.assert_array_equal(hn, hc)
+        else:
+            assert plan.info()["multi_operator"] == _lib.OPERATOR_NODE
+            assert rel_l2(hn, hc) < 1e-13
+            for npw in (1, 3, 64):
+                plan.set_tuning(_lib.TUNE_MULTI_NODES_PER_WAVE, npw)
+                h2, _ = plan.run_batched(n_t, S, src, np.arange(S), s, hi)
+                np.testing.assert_array_equal(h2, hn)
     finally:
         plan.close()
     A, B, D = ov6.abd_matrices(K, C, M, dt, mode)
