@@ -46,7 +46,8 @@ EXPORTS = (
     "bfe_assemble_fill", "bfe_assemble_last_error", "bfe_assemble_device",
     "bfe_assemble_device_fill", "bfe_assemble_device_free", "bfe_convolve",
     "bfe_disps_to_forces", "bfe_disps_to_forces_noalloc", "bfe_element_templates", "bfe_element_centres",
-    "bfe_points_in_polygon", "bfe_dist_points_to_polygon",
+    "bfe_points_in_polygon", "bfe_dist_points_to_polygon", "bfe_tri_grid",
+    "bfe_remove_unused_nodes", "bfe_get_edges",
 )
 
 
@@ -191,6 +192,13 @@ def load() -> C.CDLL:
     lib.bfe_points_in_polygon.restype = C.c_int
     lib.bfe_dist_points_to_polygon.argtypes = [C.c_int64, _f64p, C.c_int32, _f64p, _f64p]
     lib.bfe_dist_points_to_polygon.restype = C.c_int
+    lib.bfe_tri_grid.argtypes = [C.c_int32, C.c_int64, C.c_int64, _f64p, _f64p, C.c_double, _f64p,
+                                 _i64p, _i64p]
+    lib.bfe_tri_grid.restype = C.c_int
+    lib.bfe_remove_unused_nodes.argtypes = [C.c_int64, C.c_int64, C.c_int32, _i64p, _i64p, _i64p]
+    lib.bfe_remove_unused_nodes.restype = C.c_int
+    lib.bfe_get_edges.argtypes = [C.c_int64, C.c_int32, _i64p, _i64p, _i64p, _i64p, _i64p]
+    lib.bfe_get_edges.restype = C.c_int
     _lib = lib
     return lib
 
@@ -508,6 +516,54 @@ def dist_points_to_polygon(pts, bdry):
     check(lib.bfe_dist_points_to_polygon(pts.shape[0], ptr(pts, C.c_double), bdry.shape[0],
                                          ptr(bdry, C.c_double), ptr(out, C.c_double)))
     return out
+
+
+def tri_grid(iso, x, y, shift):
+    """bfe_tri_grid: (nds, els) of the structured triangle grid on axes x, y
+    (els 1-based, the reference's block order), before any region clip."""
+    lib = load()
+    x = np.ascontiguousarray(x, np.float64).ravel()
+    y = np.ascontiguousarray(y, np.float64).ravel()
+    n_el = C.c_int64(0)
+    check(lib.bfe_tri_grid(int(iso), x.size, y.size, ptr(x, C.c_double), ptr(y, C.c_double),
+                           float(shift), None, None, C.byref(n_el)))
+    nds = np.empty((x.size * y.size, 2), np.float64)
+    els = np.empty((n_el.value, 3), np.int64)
+    check(lib.bfe_tri_grid(int(iso), x.size, y.size, ptr(x, C.c_double), ptr(y, C.c_double),
+                           float(shift), ptr(nds, C.c_double), ptr(els, C.c_int64),
+                           C.byref(n_el)))
+    return nds, els
+
+
+def remove_unused_nodes(n_nodes, els):
+    """bfe_remove_unused_nodes: (renumbered els, old_nds) with old_nds 1-based."""
+    lib = load()
+    els = np.array(els, dtype=np.int64, order="C", copy=True)
+    if els.ndim == 1:
+        els = els[:, None]
+    old = np.empty(max(int(n_nodes), 1), np.int64)
+    nu = C.c_int64(0)
+    check(lib.bfe_remove_unused_nodes(int(n_nodes), els.shape[0], els.shape[1],
+                                      ptr(els, C.c_int64), ptr(old, C.c_int64), C.byref(nu)))
+    return els, old[:nu.value].copy()
+
+
+def get_edges(els):
+    """bfe_get_edges: (ed, ptr, el) with ed (n_ed x 2) the unique sorted edges
+    and el[ptr[i]:ptr[i+1]] the elements (1-based, largest first) on edge i."""
+    lib = load()
+    els = np.ascontiguousarray(els, np.int64)
+    if els.ndim == 1:
+        els = els[:, None]
+    m = max(els.shape[0] * els.shape[1], 1)
+    ed = np.empty((m, 2), np.int64)
+    pt = np.empty(m + 1, np.int64)
+    el = np.empty(m, np.int64)
+    ne = C.c_int64(0)
+    check(lib.bfe_get_edges(els.shape[0], els.shape[1], ptr(els, C.c_int64), C.byref(ne),
+                            ptr(ed, C.c_int64), ptr(pt, C.c_int64), ptr(el, C.c_int64)))
+    n = ne.value
+    return ed[:n].copy(), pt[:n + 1].copy(), el[:pt[n]].copy()
 
 
 def element_templates(nds, els0, mat, mult, max_templates, with_jfactor, device=0):

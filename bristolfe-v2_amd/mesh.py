@@ -25,6 +25,12 @@ Reference counterparts (restated, vectorised):
   fn_gaussian_pulse                code/fn_gaussian_pulse.m:1-15
 New here: structured tetrahedral cube mesh, Hann toneburst, stable time step
 from the largest eigenvalue of M^-1 K.
+
+From 20k points or elements the heavy primitives run in libbfe_hip's
+multi-threaded C++ (bfe_tri_grid, bfe_remove_unused_nodes, bfe_get_edges,
+bfe_element_centres, bfe_points_in_polygon, bfe_dist_points_to_polygon),
+with the same results as the numpy restatements kept beside them (_*_np).
+The seeded blob (a few hundred points) stays in numpy.
 """
 from __future__ import annotations
 
@@ -48,15 +54,12 @@ def fn_rectangular_structured_mesh(bdry_pts, el_size):
     ny = int(np.ceil(abs(hi[1] - lo[1]) / el_size)) + 1
     x = np.linspace(lo[0], hi[0], nx)
     y = np.linspace(lo[1], hi[1], ny)
-    # MATLAB meshgrid + column-major numbering: node (iy, ix) -> iy + ny*ix + 1
-    X, Y = np.meshgrid(x, y)
-    nds = np.stack([X.ravel(order="F"), Y.ravel(order="F")], axis=1)
-    nn = np.arange(1, nx * ny + 1).reshape((ny, nx), order="F")
-    a = np.stack([nn[:-1, :-1].ravel(order="F"), nn[1:, 1:].ravel(order="F"),
-                  nn[1:, :-1].ravel(order="F")], axis=1)
-    b = np.stack([nn[:-1, :-1].ravel(order="F"), nn[:-1, 1:].ravel(order="F"),
-                  nn[1:, 1:].ravel(order="F")], axis=1)
-    mod = _model(nds, np.concatenate([a, b], axis=0))
+    lib = _native() if nx * ny >= _NATIVE_MIN else None
+    if lib is not None:
+        nds, els = lib.tri_grid(0, x, y, 0.0)
+    else:
+        nds, els = _rect_grid_np(x, y)
+    mod = _model(nds, els)
     inside = fn_elements_in_region(mod, bdry_pts)
     mod["els"] = mod["els"][inside]
     mod["nds"], mod["els"] = fn_remove_unused_nodes(mod["nds"], mod["els"])[:2]
@@ -73,9 +76,40 @@ def fn_isometric_structured_mesh(bdry_pts, el_size):
     ny = int(np.ceil(by / el_size / sin60)) + 1
     x = np.linspace(lo[0] - el_size / 2, hi[0] + el_size / 2, nx)
     y = np.linspace(lo[1], hi[1], ny)
+    lib = _native() if nx * ny >= _NATIVE_MIN else None
+    if lib is not None:
+        nds, els = lib.tri_grid(1, x, y, el_size / 4)
+    else:
+        nds, els = _iso_grid_np(x, y, el_size / 4)
+    mod = _model(nds, els)
+    inside = fn_elements_in_region(mod, bdry_pts)
+    mod["els"] = mod["els"][inside]
+    mod["nds"], mod["els"] = fn_remove_unused_nodes(mod["nds"], mod["els"])[:2]
+    return _model(mod["nds"], mod["els"])
+
+
+def _rect_grid_np(x, y):
+    """Grid and triangles of code/fn_rectangular_structured_mesh.m:22-43
+    before the clip (MATLAB meshgrid, column-major node numbers: node
+    (iy, ix) -> iy + ny ix + 1)."""
+    nx, ny = x.size, y.size
     X, Y = np.meshgrid(x, y)
-    X[0::2, :] += el_size / 4
-    X[1::2, :] -= el_size / 4
+    nds = np.stack([X.ravel(order="F"), Y.ravel(order="F")], axis=1)
+    nn = np.arange(1, nx * ny + 1).reshape((ny, nx), order="F")
+    a = np.stack([nn[:-1, :-1].ravel(order="F"), nn[1:, 1:].ravel(order="F"),
+                  nn[1:, :-1].ravel(order="F")], axis=1)
+    b = np.stack([nn[:-1, :-1].ravel(order="F"), nn[:-1, 1:].ravel(order="F"),
+                  nn[1:, 1:].ravel(order="F")], axis=1)
+    return nds, np.concatenate([a, b], axis=0)
+
+
+def _iso_grid_np(x, y, shift):
+    """Grid and triangles of code/fn_isometric_structured_mesh.m:22-75 before
+    the clip (even rows moved by +shift, odd rows by -shift)."""
+    nx, ny = x.size, y.size
+    X, Y = np.meshgrid(x, y)
+    X[0::2, :] += shift
+    X[1::2, :] -= shift
     nn = np.arange(1, nx * ny + 1).reshape((ny, nx), order="F")
     f = lambda A: A.ravel(order="F")
     parts = [
@@ -85,12 +119,7 @@ def fn_isometric_structured_mesh(bdry_pts, el_size):
         (nn[1:-1:2, :-1], nn[1:-1:2, 1:], nn[2::2, :-1]),
     ]
     els = np.concatenate([np.stack([f(p), f(q), f(r)], axis=1) for p, q, r in parts], axis=0)
-    nds = np.stack([f(X), f(Y)], axis=1)
-    mod = _model(nds, els)
-    inside = fn_elements_in_region(mod, bdry_pts)
-    mod["els"] = mod["els"][inside]
-    mod["nds"], mod["els"] = fn_remove_unused_nodes(mod["nds"], mod["els"])[:2]
-    return _model(mod["nds"], mod["els"])
+    return np.stack([f(X), f(Y)], axis=1), els
 
 
 def cubic_structured_tet_mesh(size, el_size, origin=(0.0, 0.0, 0.0)):
@@ -216,6 +245,18 @@ def fn_elements_in_region(mod, region):
 
 def fn_remove_unused_nodes(nds, els):
     els = np.asarray(els)
+    n = np.asarray(nds).shape[0]
+    lib = _native() if els.size >= _NATIVE_MIN else None
+    if lib is not None:
+        els2, old_nds = lib.remove_unused_nodes(n, els)
+        new_nds = np.zeros(n, dtype=np.int64)
+        new_nds[old_nds - 1] = np.arange(1, old_nds.size + 1)
+        return np.asarray(nds)[old_nds - 1], els2.reshape(els.shape), old_nds, new_nds
+    return _remove_unused_nodes_np(nds, els)
+
+
+def _remove_unused_nodes_np(nds, els):
+    els = np.asarray(els)
     in_use = np.zeros(np.asarray(nds).shape[0], dtype=bool)
     u = els[els > 0]
     in_use[u - 1] = True
@@ -298,6 +339,20 @@ def add_absorbing_layer_box(mod, inner_lo, inner_hi, thickness):
 def fn_get_edges(els):
     """Unique edges (sorted node pairs, sorted rows) and the elements on each side
     (largest element number first, 0 where there is none)."""
+    els = np.asarray(els)
+    lib = _native() if els.shape[0] >= _NATIVE_MIN else None
+    if lib is not None:
+        ed, pt, el = lib.get_edges(els)
+        cnt = np.diff(pt)
+        width = int(cnt.max()) if cnt.size else 0
+        ewe = np.zeros((ed.shape[0], width), dtype=np.int64)
+        row = np.repeat(np.arange(ed.shape[0]), cnt)
+        ewe[row, np.arange(el.size) - pt[row]] = el
+        return ed, ewe
+    return _get_edges_np(els)
+
+
+def _get_edges_np(els):
     els = np.asarray(els)
     eds, owner = [], []
     nper = (els > 0).sum(axis=1)

@@ -547,6 +547,32 @@ bfe_status bfe_points_in_polygon(int64_t n_pts, const double* pts, int32_t n_v,
                                  const double* vx, const double* vy, uint8_t* out);
 bfe_status bfe_dist_points_to_polygon(int64_t n_pts, const double* pts, int32_t n_b,
                                       const double* bdry, double* out);
+/*
+ *   bfe_tri_grid               the node grid and triangles of
+ *                              code/fn_rectangular_structured_mesh.m:22-43
+ *                              (iso = 0) and code/fn_isometric_structured_mesh.m:22-75
+ *                              (iso = 1) before their region clip: x (nx),
+ *                              y (ny) the linspace axes, shift = el_size / 4
+ *                              (isometric row offset); node k = iy + ny ix;
+ *                              nds (nx ny x 2), els (n_el x 3, 1-based) in the
+ *                              reference's order; with nds = els = NULL only
+ *                              *n_el is set
+ *   bfe_remove_unused_nodes    code/internal/fn_remove_unused_nodes.m: renumbers
+ *                              els in place (used nodes in increasing order),
+ *                              old_nds (n_nodes) receives each new node's old
+ *                              number, *n_used their count
+ *   bfe_get_edges              code/fn_get_edges.m:1-35: the unique edges
+ *                              (lower, higher node; ed, *n_ed x 2) in
+ *                              increasing order and the elements on edge i,
+ *                              largest first, at el[ptr[i] .. ptr[i+1]);
+ *                              ed, ptr and el hold n_el x nper entries at most
+ */
+bfe_status bfe_tri_grid(int32_t iso, int64_t nx, int64_t ny, const double* x, const double* y,
+                        double shift, double* nds, int64_t* els, int64_t* n_el);
+bfe_status bfe_remove_unused_nodes(int64_t n_nodes, int64_t n_el, int32_t nper, int64_t* els,
+                                   int64_t* old_nds, int64_t* n_used);
+bfe_status bfe_get_edges(int64_t n_el, int32_t nper, const int64_t* els, int64_t* n_ed,
+                         int64_t* ed, int64_t* ptr, int64_t* el);
 
 bfe_status bfe_info_get(bfe_handle h, bfe_info* info);
 bfe_status bfe_last_run_stats(bfe_handle h, bfe_run_stats* stats);
