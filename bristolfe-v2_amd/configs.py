@@ -86,13 +86,26 @@ def steel_3d():
 
 
 def cube_3d(size=20e-3, el_size=0.1e-3, layer=2e-3, n_t=2000, centre_freq=2e6, cycles=3,
-            n_mon=16, mon_pitch=1.5e-3, dt=None, assembly_backend="auto"):
+            n_mon=16, mon_pitch=1.5e-3, dt=None, assembly_backend="auto", vertical=2):
+    """Steel cube of C3D4 tets (cubic_structured_tet_mesh: nodes numbered x
+    fastest, then y, then z) with absorbing layers on the four faces parallel
+    to the vertical axis, a normal toneburst force on a 3 x 3-node patch at
+    the centre of the top face (the face at the top of the vertical axis) and
+    n_mon monitors on a square grid around it on that face. vertical = 2 (the
+    default): layers on the x and y faces, top face z = size. vertical = 0
+    makes x, the fastest node index, the vertical axis (layers on the y and z
+    faces): the same physical model, rotated, whose node runs along x then
+    end on the two free x faces instead of in the layers."""
     matls = [steel_3d()]
     mod = _m.cubic_structured_tet_mesh(size, el_size)
     n_el = mod["els"].shape[0]
     mod["el_typ_i"] = np.full(n_el, "C3D4", dtype=object)
-    mod = _m.add_absorbing_layer_box(mod, (layer, layer, -np.inf), (size - layer, size - layer, np.inf),
-                                     layer)
+    if vertical not in (0, 2):
+        raise ValueError("vertical axis must be 0 (x) or 2 (z)")
+    lo = [layer] * 3
+    hi = [size - layer] * 3
+    lo[vertical], hi[vertical] = -np.inf, np.inf
+    mod = _m.add_absorbing_layer_box(mod, tuple(lo), tuple(hi), layer)
     opts = {"dof_to_use": [], "damping_power_law": 2, "assembly_backend": assembly_backend}
     K, C, M, gl = _assemble(mod, matls, opts)
     if dt is None:
@@ -100,22 +113,31 @@ def cube_3d(size=20e-3, el_size=0.1e-3, layer=2e-3, n_t=2000, centre_freq=2e6, c
     t = np.arange(n_t) * dt
     sig = _m.hann_toneburst(t, centre_freq, cycles)
     nds = mod["nds"]
-    # nodes are numbered x fastest, then y, then z: the top face is the last
-    # (nx * ny) nodes
     nc = int(round(size / el_size))
     nx = nc + 1
-    top0 = nx * nx * nc
+
+    def node(i, j, k):  # grid indices along x, y, z -> 0-based node
+        return i + nx * (j + nx * k)
+
+    # in-face axes (a, b) of the top face, the vertical index at its top
+    a_ax, b_ax = [d for d in range(3) if d != vertical]
+
+    def face_node(ia, ib):
+        g = [0, 0, 0]
+        g[vertical], g[a_ax], g[b_ax] = nc, ia, ib
+        return node(*g)
+
     ic = nc // 2
-    patch = np.array([top0 + (ic + a) + nx * (ic + b) for b in (-1, 0, 1) for a in (-1, 0, 1)])
-    assert np.allclose(nds[patch, 2], size)
-    frc = gl[patch, 2]
+    patch = np.array([face_node(ic + a, ic + b) for b in (-1, 0, 1) for a in (-1, 0, 1)])
+    assert np.allclose(nds[patch, vertical], size)
+    frc = gl[patch, vertical]
     k = int(round(np.sqrt(n_mon)))
     off = (np.arange(k) - (k - 1) / 2) * mon_pitch
-    X, Y = np.meshgrid(size / 2 + off, size / 2 + off)
-    ii = np.rint(X.ravel() / el_size).astype(np.int64)
-    jj = np.rint(Y.ravel() / el_size).astype(np.int64)
-    mon_nodes = top0 + ii + nx * jj
-    mon = gl[mon_nodes, 2]
+    A, B = np.meshgrid(size / 2 + off, size / 2 + off)
+    ii = np.rint(A.ravel() / el_size).astype(np.int64)
+    jj = np.rint(B.ravel() / el_size).astype(np.int64)
+    mon_nodes = np.array([face_node(p, q) for p, q in zip(ii, jj)], dtype=np.int64)
+    mon = gl[mon_nodes, vertical]
     return dict(mod=mod, matls=matls, fe_options=opts, K=K, C=C, M=M, gl_lookup=gl, time=t, dt=dt,
                 frc_gi=frc, frcs=sig[None, :], hist_gi=mon, src_node=patch + 1,
                 name=f"cube_3d steel {size*1e3:g} mm C3D4 h={el_size*1e3:g} mm")

@@ -18,7 +18,9 @@ fully specified inputs:
   coupled_small          aluminium under water with an interface and an
                          absorbing layer (CSR), two forcing rows with
                          leading zeros, 'KE' field frames every 50 steps
-  tet_cube_small         a 4 mm C3D4 cube at 1 mm with side absorbing layers
+  tet_cube_small         a 4 mm C3D4 cube at 1 mm with side absorbing layers,
+                         at half the CFL step (where both solver modes are
+                         stable)
 
 Each file holds the inputs (K, C, m = diag M, dt, n_t, forcing rows and
 1-based DOFs, monitored 1-based DOFs) and the histories of both solver modes,
@@ -110,12 +112,17 @@ def coupled_small():
 
 
 def tet_cube_small():
+    """At half the CFL step: with dt c/m above 2 in the layers at the full
+    step, the 'vel at last half time step' recurrence (gamma = dt c/m - 1)
+    diverges, and a fixture of two diverging runs pins nothing."""
     from bristolfe_v2_amd import configs
-    c = configs.cube_3d(size=4e-3, el_size=1e-3, layer=1e-3, n_t=150, n_mon=4, mon_pitch=1e-3,
-                        assembly_backend="numpy")
+    c0 = configs.cube_3d(size=4e-3, el_size=1e-3, layer=1e-3, n_t=2, n_mon=4, mon_pitch=1e-3,
+                         assembly_backend="numpy")
+    c = configs.cube_3d(size=4e-3, el_size=1e-3, layer=1e-3, n_t=300, n_mon=4, mon_pitch=1e-3,
+                        assembly_backend="numpy", dt=0.5 * c0["dt"])
     K, C, M = c["K"], c["C"], c["M"]
     hi = np.unique(np.concatenate([c["hist_gi"], np.arange(1, K.shape[0] + 1, 11)]))
-    _save("tet_cube_small", K, C, M.diagonal(), c["dt"], 150, c["frc_gi"], c["frcs"], hi)
+    _save("tet_cube_small", K, C, M.diagonal(), c["dt"], 300, c["frc_gi"], c["frcs"], hi)
 
 
 if __name__ == "__main__":
