@@ -805,14 +805,16 @@ struct ClassOffsets {
 // Where slot k's element keeps its scale in the anchor table (n_rows x
 // n_nodes, row = the element's template, column = its canonical first
 // vertex): entry v + base[k] on a regular node v, base[k] = row * n_nodes +
-// the node offset of the first vertex; len = the table's length. Only
+// the node offset of the first vertex; len = the table's length, below
+// 2^31 (32-bit offsets: 64-bit ones cost the triangle kernels more scalar
+// spills, 76 against 98 in the two-DOF multi-source kernel). Only
 // patterns of few slots take it: the per-slot kernel arguments compete with
 // the template rows for scalar registers (24 slots spill).
 constexpr int kAnchorMaxSlots = 8;
 template <int NS>
 struct AnchorSlots {
-  int64_t base[NS];
-  int64_t len;
+  int32_t base[NS];
+  int32_t len;
 };
 
 // Slots whose scale reads a batch of the slot product issues together (on
@@ -2562,8 +2564,9 @@ void launch_ebe_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& Gr, int
   const bool anchored = Gr.anchor.bytes > 0;  // only on groups whose slots are mostly scaled
   if (anchored) {
     for (int k = 0; k < PT::NS; ++k)
-      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
-    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+      an.base[k] = static_cast<int32_t>(static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes +
+                                        Gr.anc_noff[k]);
+    an.len = static_cast<int32_t>(Gr.anchor.bytes / sizeof(T));
   }
   const uint32_t* sm = Gr.smask.as<uint32_t>();
   if constexpr (PT::NS <= kAnchorMaxSlots) {
@@ -3435,8 +3438,9 @@ void launch_ebe_multi_pat(const bfe_plan_s& H, const EbeOp& E, const EbeGroup& G
   const bool anchored = PT::NS <= kAnchorMaxSlots && Gr.anchor.bytes > 0;
   if (anchored) {
     for (int k = 0; k < PT::NS; ++k)
-      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
-    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+      an.base[k] = static_cast<int32_t>(static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes +
+                                        Gr.anc_noff[k]);
+    an.len = static_cast<int32_t>(Gr.anchor.bytes / sizeof(T));
   }
   const int32_t slide = (H.multi_pipeline && offsets_follow_next<PT>(Gr.doff, DPN)) ? 1 : 0;
   const int64_t nblk = ebe_multi_blocks(Gr, npw);
@@ -3513,8 +3517,9 @@ void build_node_blocks_pat(const bfe_plan_s& H, EbeGroup& Gr) {
   const bool anchored = PT::NS <= kAnchorMaxSlots && Gr.anchor.bytes > 0;
   if (anchored) {
     for (int k = 0; k < PT::NS; ++k)
-      an.base[k] = static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes + Gr.anc_noff[k];
-    an.len = static_cast<int64_t>(Gr.anchor.bytes / sizeof(T));
+      an.base[k] = static_cast<int32_t>(static_cast<int64_t>(Gr.anc_row[k]) * Gr.n_nodes +
+                                        Gr.anc_noff[k]);
+    an.len = static_cast<int32_t>(Gr.anchor.bytes / sizeof(T));
   }
   Gr.nblk = DevBuf(sizeof(T) * static_cast<size_t>(Gr.n_nodes) * PT::NC * DPN * DPN);
   const dim3 grid((Gr.n_nodes + kBlock - 1) / kBlock), blk(kBlock);
@@ -4037,6 +4042,8 @@ void set_element_groups(bfe_plan_s* h, int32_t n_groups, const bfe_element_group
           }
         }
         std::vector<double> anc;
+        // (the kernels index the table with 32-bit offsets)
+        ok = ok && static_cast<int64_t>(n_rows) * nn < (int64_t(1) << 31);
         if (ok) {
           anc.assign(static_cast<size_t>(n_rows) * nn, 0.0);
           std::vector<int32_t> taken(static_cast<size_t>(n_rows) * nn, -1);
