@@ -34,12 +34,15 @@ TUNE_MULTI_OPERATOR = 11
 TUNE_MULTI_NODES_PER_WAVE = 12
 TUNE_MULTI_SLIDE = 13
 TUNE_ADJOINT_PAIRS = 14
+# per-element field types of bfe_run_field
+FIELD_KE, FIELD_MEAN, FIELD_DIV, FIELD_CURL = 0, 1, 2, 3
 
 # Every symbol the header declares.
 EXPORTS = (
     "bfe_create", "bfe_run", "bfe_run_batched", "bfe_set_gradient_elements",
     "bfe_run_checkpointed", "bfe_run_checkpointed_batched", "bfe_adjoint", "bfe_set_tuning", "bfe_set_element_operator",
     "bfe_set_element_groups", "bfe_set_field_elements", "bfe_run_field_ke",
+    "bfe_set_field_geometry", "bfe_run_field",
     "bfe_set_kernel_timing",
     "bfe_info_get",
     "bfe_last_run_stats", "bfe_destroy", "bfe_last_error", "bfe_assemble_pattern",
@@ -142,6 +145,12 @@ def load() -> C.CDLL:
     lib.bfe_run_field_ke.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32,
                                      _i32p, C.c_int32, _f64p, C.c_int32, _f64p, _i32p]
     lib.bfe_run_field_ke.restype = C.c_int
+    lib.bfe_set_field_geometry.argtypes = [C.c_void_p, C.c_int32, _f64p]
+    lib.bfe_set_field_geometry.restype = C.c_int
+    lib.bfe_run_field.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, C.c_int32,
+                                  _i32p, C.c_int32, _f64p, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32, _f64p, _i32p]
+    lib.bfe_run_field.restype = C.c_int
     lib.bfe_set_kernel_timing.argtypes = [C.c_void_p, C.c_int32]
     lib.bfe_set_kernel_timing.restype = C.c_int
     lib.bfe_info_get.argtypes = [C.c_void_p, _p(BfeInfo)]
@@ -333,9 +342,27 @@ class Plan:
                                                ptr(nd0, C.c_int32), els0.shape[0], els0.shape[1],
                                                ptr(els0, C.c_int32)))
 
+    def set_field_geometry(self, nds):
+        """Node coordinates (n_nodes, dim) for the 'div' and 'curl' fields."""
+        nds = np.ascontiguousarray(nds, np.float64)
+        check(self._lib.bfe_set_field_geometry(self._h, int(nds.shape[1]), ptr(nds, C.c_double)))
+        self.field_dim = int(nds.shape[1])
+
+    def run_field(self, n_t, frc_idx, frcs, hist_idx, field_every, field_type, field_is_vel,
+                  comp=0):
+        """bfe_run with a per-element field reduced on the device (FIELD_KE,
+        FIELD_MEAN of DOF column comp, FIELD_DIV, FIELD_CURL, from velocity
+        or displacement frames): returns (hist, el (n_el x n_frames),
+        ti_start)."""
+        return self._run_field(n_t, frc_idx, frcs, hist_idx, field_every,
+                               (int(field_type), 1 if field_is_vel else 0, int(comp)))
+
     def run_field_ke(self, n_t, frc_idx, frcs, hist_idx, field_every):
         """bfe_run with the 'KE' field reduced on the device: returns (hist,
         el_ke (n_el x n_frames), ti_start)."""
+        return self._run_field(n_t, frc_idx, frcs, hist_idx, field_every, None)
+
+    def _run_field(self, n_t, frc_idx, frcs, hist_idx, field_every, kind):
         frc_idx = np.ascontiguousarray(frc_idx, np.int32).ravel()
         hist_idx = np.ascontiguousarray(hist_idx, np.int32).ravel()
         p, q = frc_idx.size, hist_idx.size
@@ -348,10 +375,16 @@ class Plan:
         n_frames = (n_t - 1) // int(field_every) + 1
         el = np.empty((self.n_field_el, n_frames), order="F")
         ti = C.c_int32()
-        check(self._lib.bfe_run_field_ke(
-            self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
-            ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(field_every),
-            el.ctypes.data_as(_f64p), C.byref(ti)))
+        if kind is None:
+            check(self._lib.bfe_run_field_ke(
+                self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
+                ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(field_every),
+                el.ctypes.data_as(_f64p), C.byref(ti)))
+        else:
+            check(self._lib.bfe_run_field(
+                self._h, int(n_t), ptr(frc_idx, C.c_int32), p, Fcol.ctypes.data_as(_f64p), rows,
+                ptr(hist_idx, C.c_int32), q, hist.ctypes.data_as(_f64p), int(field_every),
+                kind[0], kind[1], kind[2], el.ctypes.data_as(_f64p), C.byref(ti)))
         return hist, el, int(ti.value)
 
     def set_element_groups(self, groups, check_tol=0.0):

@@ -1405,6 +1405,73 @@ __global__ void k_field_ke(int64_t n_el, int32_t npe, const int32_t* __restrict_
   out[i] = cnt > 0 ? sqrt(sum / T(cnt)) : T(0);
 }
 
+// The other per-element fields of code/internal/fn_get_field_output.m:28-58
+// from displacement or velocity frames (n x F), one thread per (element,
+// frame). BFE_FIELD_MEAN: the mean over the element's nodes of DOF column
+// comp (an absent DOF counts as 0, :35-37 and :118-134). BFE_FIELD_DIV /
+// _CURL: the sum over the element's nodes of the node's components 1..dim
+// dotted with the unit vector from the element centre to the node (:85-98;
+// for CURL turned to (y, -x), :100-105); 0 on an element with an absent
+// node (:95). Centres, lengths and sums in the reference's order, without
+// FMA contraction, so the host restatement's values come out bit for bit.
+template <typename T>
+__global__ void k_field_reduce(int32_t type, int32_t comp, int64_t n_el, int32_t npe,
+                               const int32_t* __restrict__ els, int32_t max_df,
+                               const int32_t* __restrict__ node_dof,
+                               const double* __restrict__ nds, int32_t dim,
+                               const T* __restrict__ fld, int64_t n, int32_t F,
+                               T* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n_el * F) return;
+  const int64_t e = i % n_el, f = i / n_el;
+  const T* v = fld + f * n;
+  const int32_t* en = els + e * npe;
+  auto val = [&](int32_t node, int d) -> T {
+    const int32_t g = node_dof[static_cast<int64_t>(node) * max_df + d];
+    return g >= 0 ? v[g] : T(0);
+  };
+  if (type == BFE_FIELD_MEAN) {
+    T sum = T(0);
+    int cnt = 0;
+    for (int a = 0; a < npe; ++a) {
+      if (en[a] < 0) continue;
+      sum += val(en[a], comp);
+      ++cnt;
+    }
+    out[i] = sum / T(cnt);
+    return;
+  }
+  for (int a = 0; a < npe; ++a)
+    if (en[a] < 0) {
+      out[i] = T(0);
+      return;
+    }
+  double c[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < npe; ++a)
+    for (int d = 0; d < dim; ++d) c[d] += nds[static_cast<int64_t>(en[a]) * dim + d];
+  for (int d = 0; d < dim; ++d) c[d] /= static_cast<double>(npe);
+  T sum = T(0);
+  for (int a = 0; a < npe; ++a) {
+    double w[3], len = 0.0;
+    for (int d = 0; d < dim; ++d) {
+      w[d] = nds[static_cast<int64_t>(en[a]) * dim + d] - c[d];
+      len += w[d] * w[d];
+    }
+    len = sqrt(len);
+    for (int d = 0; d < dim; ++d) w[d] /= len;
+    if (type == BFE_FIELD_CURL) {
+      const double x = w[0];
+      w[0] = w[1];
+      w[1] = -x;
+    }
+    T dot = T(0);
+    for (int d = 0; d < dim; ++d) dot += val(en[a], d) * static_cast<T>(w[d]);
+    sum += dot;
+  }
+  out[i] = sum;
+}
+
 // ---------------------------------------------------------------------------
 // Per-element gradients over one checkpoint segment of Ls time steps.
 // Thread e keeps K_e (KMAX x KMAX), the lumped mass per unit density and the
@@ -2800,6 +2867,8 @@ struct FieldElements {
   int64_t n_el = 0, n_nodes = 0;
   int32_t npe = 0, max_df = 0;
   DevBuf els, node_dof, m;
+  int32_t dim = 0;  // node coordinates (BFE_FIELD_DIV / _CURL), 0: not set
+  DevBuf nds;
 };
 
 template <typename T>
@@ -2807,7 +2876,8 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
                 const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
                 double* hist_out, int32_t field_every, int32_t field_is_vel,
                 double* field_out, int32_t* ti_start_out, AdjointRecord* rec, int32_t ckpt_every,
-                double* el_ke_out = nullptr) {
+                double* el_ke_out = nullptr, int32_t field_type = BFE_FIELD_KE,
+                int32_t field_comp = 0) {
   const int64_t n = H.n;
   validate_run(n, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out);
   const int32_t n_frames = field_every > 0 ? (n_t - 1) / field_every + 1 : 0;
@@ -2961,9 +3031,15 @@ void run_single(bfe_plan_s& H, int32_t n_t, const int32_t* frc_idx, int32_t p,
     // per-element field on the device: only n_el x F values come back
     const int64_t work = FE->n_el * n_frames;
     DevBuf dke(sizeof(T) * static_cast<size_t>(work));
-    hipLaunchKernelGGL((k_field_ke<T>), dim3((work + 255) / 256), dim3(256), 0, st, FE->n_el,
-                       FE->npe, FE->els.as<int32_t>(), FE->max_df, FE->node_dof.as<int32_t>(),
-                       FE->m.as<T>(), fld, n, n_frames, dke.as<T>());
+    if (field_type == BFE_FIELD_KE)
+      hipLaunchKernelGGL((k_field_ke<T>), dim3((work + 255) / 256), dim3(256), 0, st, FE->n_el,
+                         FE->npe, FE->els.as<int32_t>(), FE->max_df, FE->node_dof.as<int32_t>(),
+                         FE->m.as<T>(), fld, n, n_frames, dke.as<T>());
+    else
+      hipLaunchKernelGGL((k_field_reduce<T>), dim3((work + 255) / 256), dim3(256), 0, st,
+                         field_type, field_comp, FE->n_el, FE->npe, FE->els.as<int32_t>(),
+                         FE->max_df, FE->node_dof.as<int32_t>(), FE->nds.as<double>(), FE->dim,
+                         fld, n, n_frames, dke.as<T>());
     BFE_HIP(hipGetLastError());
     BFE_HIP(hipStreamSynchronize(st));
     copy_out<T>(dke, el_ke_out, static_cast<size_t>(work));
@@ -4283,6 +4359,52 @@ bfe_status bfe_run_field_ke(bfe_handle h, int32_t n_t, const int32_t* frc_idx, i
     else
       run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, field_every,
                         1, nullptr, ti_start_out, nullptr, 0, el_out);
+  });
+}
+
+bfe_status bfe_set_field_geometry(bfe_handle h, int32_t dim, const double* nds) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    auto F = std::static_pointer_cast<FieldElements>(h->field_elems);
+    if (!F) throw std::invalid_argument("field elements not set");
+    if (dim < 1 || dim > 3) throw std::invalid_argument("dim must be 1, 2 or 3");
+    if (dim > F->max_df) throw std::invalid_argument("dim exceeds the DOFs per node");
+    if (nds == nullptr) throw std::invalid_argument("NULL node coordinates");
+    for (int64_t i = 0; i < F->n_nodes * dim; ++i)
+      if (!std::isfinite(nds[i])) throw std::invalid_argument("non-finite node coordinate");
+    BFE_HIP(hipSetDevice(h->device));
+    F->nds = upload(std::vector<double>(nds, nds + F->n_nodes * dim));
+    F->dim = dim;
+  });
+}
+
+bfe_status bfe_run_field(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                         const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
+                         double* hist_out, int32_t field_every, int32_t field_type,
+                         int32_t field_is_vel, int32_t comp, double* el_out,
+                         int32_t* ti_start_out) {
+  return guarded([&] {
+    if (h == nullptr) throw std::invalid_argument("NULL handle");
+    if (field_every <= 0) throw std::invalid_argument("field_every must be positive");
+    if (el_out == nullptr) throw std::invalid_argument("element field output is NULL");
+    auto F = std::static_pointer_cast<FieldElements>(h->field_elems);
+    if (!F) throw std::invalid_argument("field elements not set");
+    if (field_type < BFE_FIELD_KE || field_type > BFE_FIELD_CURL)
+      throw std::invalid_argument("unknown field type");
+    if (field_type == BFE_FIELD_MEAN && (comp < 0 || comp >= F->max_df))
+      throw std::invalid_argument("field component out of range");
+    if (field_type == BFE_FIELD_DIV || field_type == BFE_FIELD_CURL) {
+      if (F->dim == 0) throw std::invalid_argument("field geometry not set");
+      if (field_type == BFE_FIELD_CURL && F->dim != 2)  // fn_get_field_output.m:47-49
+        throw std::invalid_argument("Cannot output curl for 3D models");
+    }
+    const int32_t vel = field_is_vel ? 1 : 0;
+    if (h->prec == BFE_PREC_DOUBLE)
+      run_single<double>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, field_every,
+                         vel, nullptr, ti_start_out, nullptr, 0, el_out, field_type, comp);
+    else
+      run_single<float>(*h, n_t, frc_idx, p, frcs, frc_rows, hist_idx, q, hist_out, field_every,
+                        vel, nullptr, ti_start_out, nullptr, 0, el_out, field_type, comp);
   });
 }
 

@@ -21,6 +21,7 @@ import time as _time
 
 import numpy as np
 
+from . import _lib
 from .assembly import fn_build_global_matrices_v5
 from .field_output import fn_get_field_output
 from .options import BRISTOLFE_DEFAULTS, ENTRY_POINT_DEFAULTS, set_default_fields
@@ -45,6 +46,23 @@ def fn_nds_and_dfs_to_gi(nds, dfs, gl_lookup):
     gi = gl_lookup[nds - 1, dfs - 1].astype(np.int64)
     valid = gi > 0
     return gi[valid], nds[valid], dfs[valid], valid
+
+
+def device_field_kind(field_output_type, dim):
+    """(type, is_vel, comp) of the device reduction for a field_output_type
+    string (fn_get_field_output.m:28-79; frames as v6:64-69), None for
+    'raw(u)' / 'raw(v)' (the nodal field itself) and for 'curl' on a 3D model
+    (the host path raises the reference's error)."""
+    t = field_output_type
+    if t in ("raw(u)", "raw(v)"):
+        return None
+    if t in ("mean(u1)", "mean(u2)", "mean(u3)", "mean(v1)", "mean(v2)", "mean(v3)"):
+        return (_lib.FIELD_MEAN, t[5] == "v", int(t[6]) - 1)
+    if t in ("div(u)", "div(v)"):
+        return (_lib.FIELD_DIV, t[4] == "v", 0)
+    if t in ("curl(u)", "curl(v)"):
+        return (_lib.FIELD_CURL, t[5] == "v", 0) if dim == 2 else None
+    return (_lib.FIELD_KE, True, 0)  # the reference's 'otherwise' branch (:65)
 
 
 def fn_FE_entry_point(mod, matls, steps, fe_options=None, return_mats=False, verbose=False):
@@ -127,17 +145,22 @@ def fn_BristolFE_v2(mod, matls, steps, fe_options=None, return_mats=False):
                                        operator=operator)
             frames = field_frames(t.size, fe_options["field_output_every_n_frames"])
             el_fld = None
-            if (frames.size and fe_options["field_output_type"] == "KE"
-                    and fe_options.get("device_field_output", True)):
-                # 'KE' field reduced per element on the device (fn_get_field_output.m:64-79)
+            ftype = fe_options["field_output_type"]
+            kind = device_field_kind(ftype, np.asarray(mod["nds"]).shape[1])
+            if frames.size and kind is not None and fe_options.get("device_field_output", True):
+                # the per-element field reduced on the device (fn_get_field_output.m:28-79)
                 plan = plans[key]
                 if getattr(plan, "n_field_el", None) is None:
                     plan.set_field_elements(mod["els"], gl_lookup)
+                if kind[0] in (_lib.FIELD_DIV, _lib.FIELD_CURL) and \
+                        getattr(plan, "field_dim", None) is None:
+                    plan.set_field_geometry(mod["nds"])
                 F = np.asarray(frcs, np.float64)
                 F = F[None, :] if F.ndim == 1 else F
-                mon_dsps, el_fld, ti_start = plan.run_field_ke(
+                mon_dsps, el_fld, ti_start = plan.run_field(
                     t.size, np.asarray(frc_gi, np.int64) - 1, F if np.size(frc_gi) else np.zeros((1, t.size)),
-                    np.asarray(hist_gi, np.int64) - 1, int(fe_options["field_output_every_n_frames"]))
+                    np.asarray(hist_gi, np.int64) - 1, int(fe_options["field_output_every_n_frames"]),
+                    *kind)
                 fld = None
                 fld_time = np.where(frames >= ti_start, t[frames - 1], 0.0)
             else:

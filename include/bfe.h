@@ -171,6 +171,35 @@ bfe_status bfe_run_field_ke(bfe_handle h, int32_t n_t, const int32_t* frc_idx, i
                             int32_t* ti_start_out);
 
 /*
+ * The other per-element fields of code/internal/fn_get_field_output.m:28-58
+ * on the device (the field_output_type strings of v6:64-69 and
+ * fn_BristolFE_v2.m:145):
+ *   BFE_FIELD_KE    'KE' (as bfe_run_field_ke; velocity frames)
+ *   BFE_FIELD_MEAN  'mean(ui)' / 'mean(vi)': element mean of DOF column
+ *                   comp (0-based column of gl_lookup; an absent DOF is 0)
+ *   BFE_FIELD_DIV   'div(u)' / 'div(v)': sum over the element's nodes of
+ *                   components 1..dim dotted with the unit vector from the
+ *                   element centre to the node; 0 on an element with an
+ *                   absent node
+ *   BFE_FIELD_CURL  'curl(u)' / 'curl(v)': as DIV with the unit vectors
+ *                   turned to (y, -x); 2D only (the reference errors on 3D)
+ * field_is_vel picks velocity (1) or displacement (0) frames, as v6 does
+ * from the type string. DIV and CURL need bfe_set_field_geometry: the node
+ * coordinates, n_nodes x dim row-major (dim <= the DOFs per node of
+ * bfe_set_field_elements). el_out is n_el x n_frames (column-major).
+ */
+#define BFE_FIELD_KE 0
+#define BFE_FIELD_MEAN 1
+#define BFE_FIELD_DIV 2
+#define BFE_FIELD_CURL 3
+bfe_status bfe_set_field_geometry(bfe_handle h, int32_t dim, const double* nds);
+bfe_status bfe_run_field(bfe_handle h, int32_t n_t, const int32_t* frc_idx, int32_t p,
+                         const double* frcs, int32_t frc_rows, const int32_t* hist_idx, int32_t q,
+                         double* hist_out, int32_t field_every, int32_t field_type,
+                         int32_t field_is_vel, int32_t comp, double* el_out,
+                         int32_t* ti_start_out);
+
+/*
  * bfe_run_batched — march n_src independent load cases on the same operator
  * in one launch sequence (sparse x multivector). Replaces the sequential
  * per-step loop of code/fn_BristolFE_v2.m:72-123 as driven by

@@ -457,6 +457,104 @@ def test_device_ke_field_matches_host_reduction(prec):
     assert rel_l2(el, ke) < tol
 
 
+FIELD_TYPES = ["mean(u1)", "mean(v2)", "mean(u4)", "div(u)", "div(v)", "curl(u)", "curl(v)"]
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+@pytest.mark.parametrize("ftype", FIELD_TYPES)
+def test_device_element_fields_match_host_reduction(ftype, prec):
+    """bfe_run_field: the 'mean', 'div' and 'curl' fields reduced per element
+    on the device. Coupled model with a void: CPE3 and AC2D3 elements, and
+    2-node ASI2D2 rows (an absent third node: 0 for div and curl). In FP64
+    the device values equal the host restatement (field_output.py, pinned by
+    closed forms) applied to the same run's frames bit for bit, and the
+    oracle's frames reduced on the host to 1e-10."""
+    print("case: device element field", ftype, prec, flush=True)
+    from bristolfe_v2_amd import driver
+    void = PKG.mesh.fn_create_smooth_random_blob(0.5, 3, 48, 7) * 0.3e-3 + np.array([1.5e-3, 0.7e-3])
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.1e-3, void=void)
+    opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2}
+    K, C, M, gl, t, F, fi, hi = _setup(mod, matls, opts, 400, 5e6, src=[1.5e-3, 2.0e-3], dof=4)
+    kind = driver.device_field_kind(ftype, 2)
+    mats = {"M": M, "gl_lookup": gl}
+    plan = PKG.make_plan(K, C, M, t[1] - t[0], solver_precision=prec)
+    try:
+        plan.set_field_elements(mod["els"], gl)
+        plan.set_field_geometry(mod["nds"])
+        h, el, ti = plan.run_field(t.size, fi - 1, F, hi - 1, 40, *kind)
+        h2, fld, ti2 = plan.run(t.size, fi - 1, F, hi - 1, 40, 1 if kind[1] else 0)
+    finally:
+        plan.close()
+    host = PKG.fn_get_field_output(fld, mod, mats, ftype)
+    assert ti == ti2 and el.shape == host.shape and np.linalg.norm(host) > 0
+    np.testing.assert_array_equal(h, h2)
+    if ftype.startswith(("div", "curl")):
+        invalid = np.any(np.asarray(mod["els"]) == 0, axis=1)
+        assert invalid.any() and not el[invalid].any()
+    if prec == "double":
+        np.testing.assert_array_equal(el, host)
+    else:
+        assert rel_l2(el, host) < 1e-5
+    r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi, F, hi, 40, ftype)
+    ref = PKG.fn_get_field_output(rf, mod, mats, ftype)
+    assert ti == rti
+    assert rel_l2(el, ref) < (TOL if prec == "double" else 1e-3)
+
+
+def test_device_element_field_errors():
+    """Argument checks of bfe_set_field_geometry / bfe_run_field."""
+    print("case: device element field errors", flush=True)
+    from bristolfe_v2_amd import _lib, configs
+    mod, matls = coupled_block(3e-3, 1.5e-3, 1.0e-3, 0.25e-3)
+    opts = {"dof_to_use": [1, 2, 4]}
+    K, C, M, gl, t, F, fi, hi = _setup(mod, matls, opts, 50, 5e6, src=[1.5e-3, 2.0e-3], dof=4)
+    with PKG.make_plan(K, C, M, t[1] - t[0]) as plan:
+        with pytest.raises(_lib.BfeError):  # no element tables yet
+            plan.set_field_geometry(mod["nds"])
+        plan.set_field_elements(mod["els"], gl)
+        with pytest.raises(_lib.BfeError, match="geometry"):
+            plan.run_field(t.size, fi - 1, F, hi - 1, 10, _lib.FIELD_DIV, 0)
+        with pytest.raises(_lib.BfeError, match="component"):
+            plan.run_field(t.size, fi - 1, F, hi - 1, 10, _lib.FIELD_MEAN, 0, 4)
+        with pytest.raises(_lib.BfeError, match="unknown field type"):
+            plan.run_field(t.size, fi - 1, F, hi - 1, 10, 7, 0)
+    c = configs.cube_3d(el_size=1e-3, n_t=20)
+    with PKG.make_plan(c["K"], c["C"], c["M"], c["dt"]) as plan:
+        plan.set_field_elements(c["mod"]["els"], c["gl_lookup"])
+        plan.set_field_geometry(c["mod"]["nds"])
+        with pytest.raises(_lib.BfeError, match="curl for 3D"):
+            plan.run_field(20, c["frc_gi"] - 1, c["frcs"], c["hist_gi"] - 1, 10, _lib.FIELD_CURL, 0)
+        h, el, _ = plan.run_field(20, c["frc_gi"] - 1, c["frcs"], c["hist_gi"] - 1, 10,
+                                  _lib.FIELD_DIV, 0)
+        assert el.shape == (c["mod"]["els"].shape[0], 2) and np.isfinite(el).all()
+
+
+def test_driver_device_fields_match_host_path():
+    """fn_FE_entry_point with each field_output_type: the device reduction
+    (default) against device_field_output = False (frames downloaded, host
+    reduction), bit for bit."""
+    print("case: driver device fields", flush=True)
+    mod, matls = coupled_block(2e-3, 1.0e-3, 0.75e-3, 0.125e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4]})
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    t = np.arange(200) * dt
+    nd = PKG.mesh.fn_find_node_at_point(mod["nds"], [1e-3, 1.5e-3])
+    steps = {"load": {"time": t, "frcs": PKG.mesh.hann_toneburst(t, 5e6, 3), "frc_nds": [nd],
+                      "frc_dfs": [4]},
+             "mon": {"nds": [nd], "dfs": [4]}}
+    for ftype in ["KE", "mean(u2)", "div(v)", "curl(u)"]:
+        opts = {"dof_to_use": [1, 2, 4], "field_output_every_n_frames": 25,
+                "field_output_type": ftype}
+        dev = PKG.fn_FE_entry_point(mod, matls, steps, opts)[0]
+        host = PKG.fn_FE_entry_point(mod, matls, steps, dict(opts, device_field_output=False))[0]
+        assert np.linalg.norm(dev["fld"]) > 0
+        np.testing.assert_array_equal(dev["fld_time"], host["fld_time"])
+        if ftype == "KE":  # sqrt of a sum: the device and host orders differ in rounding
+            assert rel_l2(dev["fld"], host["fld"]) < 1e-14
+        else:
+            np.testing.assert_array_equal(dev["fld"], host["fld"])
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_batched_row_groups_tiny_model(mode):
     """A 9-DOF model (one steel and one water triangle with their interface;
