@@ -338,6 +338,7 @@ class Plan:
         els0 = np.ascontiguousarray(np.asarray(els, np.int64) - 1, np.int32)
         nd0 = np.ascontiguousarray(np.asarray(gl_lookup, np.int64) - 1, np.int32)
         self.n_field_el = els0.shape[0]
+        self.field_dim = None  # new tables drop the node coordinates
         check(self._lib.bfe_set_field_elements(self._h, nd0.shape[0], nd0.shape[1],
                                                ptr(nd0, C.c_int32), els0.shape[0], els0.shape[1],
                                                ptr(els0, C.c_int32)))
