@@ -518,6 +518,11 @@ def test_device_element_field_errors():
             plan.run_field(t.size, fi - 1, F, hi - 1, 10, _lib.FIELD_MEAN, 0, 4)
         with pytest.raises(_lib.BfeError, match="unknown field type"):
             plan.run_field(t.size, fi - 1, F, hi - 1, 10, 7, 0)
+        # an all-zero forcing: v6's loop is empty (v6:163-180), every frame is 0
+        h, el, ti = plan.run_field(t.size, fi - 1, np.zeros_like(F), hi - 1, 10, _lib.FIELD_MEAN,
+                                   0, 1)
+        assert ti == t.size + 1 and not h.any() and not el.any()
+        assert el.shape == (mod["els"].shape[0], (t.size - 1) // 10 + 1)
     c = configs.cube_3d(el_size=1e-3, n_t=20)
     with PKG.make_plan(c["K"], c["C"], c["M"], c["dt"]) as plan:
         plan.set_field_elements(c["mod"]["els"], c["gl_lookup"])
