@@ -169,8 +169,11 @@ bool dense_solve_unit(std::vector<double>& A, int s, int c, std::vector<double>&
 
 }  // namespace
 
-Csr truncated_inverse(const Csr& X, double drop_tol) {
+Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& w) {
   const int64_t nI = X.n;
+  if (!w.empty() && static_cast<int64_t>(w.size()) != nI)
+    throw std::invalid_argument("truncated_inverse: weight count differs from the block size");
+  auto wt = [&](int32_t g) { return w.empty() ? 1.0 : w[g]; };
   // symmetric adjacency of X's pattern (off-diagonal)
   std::vector<std::vector<int32_t>> adj(nI);
   for (int64_t i = 0; i < nI; ++i)
@@ -249,8 +252,9 @@ Csr truncated_inverse(const Csr& X, double drop_tol) {
       }
       double xmax = 0.0, xbdry = 0.0;
       for (int a = 0; a < s; ++a) {
-        xmax = std::max(xmax, std::fabs(x[a]));
-        if (depth[a] == radius) xbdry = std::max(xbdry, std::fabs(x[a]));
+        const double xa = std::fabs(x[a]) * wt(order[a]);
+        xmax = std::max(xmax, xa);
+        if (depth[a] == radius) xbdry = std::max(xbdry, xa);
       }
       if (!std::isfinite(xmax)) {
         if (!failed.exchange(true)) fail_msg = "non-finite interface inverse";
@@ -259,7 +263,7 @@ Csr truncated_inverse(const Csr& X, double drop_tol) {
       if (exhausted || xbdry <= 1e-20 * xmax) {
         auto& c = cols[l];
         for (int a = 0; a < s; ++a)
-          if (std::fabs(x[a]) > drop_tol * xmax) c.emplace_back(order[a], x[a]);
+          if (std::fabs(x[a]) * wt(order[a]) > drop_tol * xmax) c.emplace_back(order[a], x[a]);
         return;
       }
       radius *= 2;
@@ -441,7 +445,9 @@ Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C, doubl
         }
         X.ptr[k + 1] = static_cast<int64_t>(X.col.size());
       }
-      P.xinv = truncated_inverse(X, 1e-18);
+      std::vector<double> w(nI);  // balance the block's very different row scales
+      for (int64_t k = 0; k < nI; ++k) w[k] = std::sqrt(std::fabs(m[P.iface[k]]));
+      P.xinv = truncated_inverse(X, 1e-18, w);
       P.has_xinv = true;
     } else {
       P.sq = -dt;

@@ -91,7 +91,11 @@ Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C,
 // by column on growing graph neighbourhoods (dense LU with partial pivoting
 // on each neighbourhood) until the neighbourhood boundary carries less than
 // 1e-20 of the column; entries below drop_tol of a column's largest entry
-// are dropped. Exposed for tests.
-Csr truncated_inverse(const Csr& X, double drop_tol);
+// are dropped. Both tests weigh entry (a, l) by w[a] (w empty: 1), so that
+// rows of very different scale are judged alike: with X = I + dt/2 M^-1 C
+// and w = sqrt|m| the weighted entries are those of the balanced
+// |M|^1/2 X^-1 |M|^-1/2 (on the fluid-solid interface |m| spans 13 orders
+// of magnitude, and unweighted dropping cost 1e-10 of the histories).
+Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& w = {});
 
 }  // namespace bfe

@@ -1,0 +1,109 @@
+"""Seeded random cases on the GPU against the oracle's literal v6: mesh kind
+(right-triangle plate, isometric plate, fluid-solid block with interface
+rows), size, absorbing layer, void, solver mode, forcing layout (one row
+broadcast to several entries, or one row per entry with leading zeros and
+duplicate entries, the last duplicate winning), monitors (with duplicates)
+and field output. Each case runs the CSR operator and the element operator
+(FP64, 1e-10 relative L2 on histories and field frames) and the batched
+march (one column per forcing entry) against single-source runs (1e-13)
+and the oracle (1e-10)."""
+import numpy as np
+import pytest
+
+from conftest import PKG, assemble, coupled_block, rect, rel_l2, steel
+from oracle import v6 as ov6
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["vel at curent time step", "vel at last half time step"]
+TOL = 1e-10
+
+
+def _case(seed):
+    rng = np.random.default_rng(seed)
+    m = PKG.mesh
+    kind = ["plate", "iso", "coupled"][seed % 3]
+    el = float(rng.choice([0.05e-3, 0.1e-3, 0.125e-3, 0.2e-3]))
+    if seed >= 9:
+        el = 0.03e-3  # tens of thousands of DOF: whole wavefronts of regular nodes
+    w = float(rng.uniform(1.5e-3, 5e-3))
+    if kind == "coupled":
+        hs, hf = float(rng.uniform(0.8e-3, 1.6e-3)), float(rng.uniform(0.5e-3, 1.0e-3))
+        void = None
+        if rng.random() < 0.5:
+            void = m.fn_create_smooth_random_blob(0.5, 3, 48, int(seed)) * 0.25e-3 + \
+                np.array([w / 2, hs / 2])
+        mod, matls = coupled_block(w, hs, hf, el, absorbing=bool(rng.random() < 0.7), void=void)
+        opts = {"dof_to_use": [1, 2, 4], "damping_power_law": int(rng.choice([2, 3]))}
+    else:
+        h = float(rng.uniform(1e-3, 2.5e-3))
+        poly = rect(w, h)
+        if kind == "iso":
+            mod = m.fn_isometric_structured_mesh(poly, el)
+        else:
+            mod = m.fn_rectangular_structured_mesh(poly, el)
+        mod["el_typ_i"] = np.array(["CPE3"] * len(mod["el_mat_i"]), dtype=object)
+        if rng.random() < 0.7:
+            t_l = float(rng.uniform(0.2, 0.35)) * min(w, h)
+            inner = np.array([[t_l, t_l], [w - t_l, t_l], [w - t_l, 10 * h], [t_l, 10 * h]])
+            mod = m.fn_add_absorbing_layer(mod, inner, t_l)
+        matls = [steel()]
+        opts = {"dof_to_use": [1, 2], "damping_power_law": int(rng.choice([2, 3]))}
+    K, C, M, gl = assemble(mod, matls, opts)
+    n = K.shape[0]
+    dt = m.stable_time_step(K, M.diagonal())
+    if opts["damping_power_law"] and C.nnz:
+        dt *= 0.5  # both solver modes stable in the layers (dt c / m stays below 2)
+    n_t = int(rng.integers(150, 300))
+    t = np.arange(n_t) * dt
+    p = int(rng.integers(1, 5))
+    fi = rng.choice(n, size=p, replace=False)
+    if p > 1 and rng.random() < 0.5:
+        fi[-1] = fi[0]  # a duplicate entry: the later one wins (v6:182)
+    if rng.random() < 0.4:
+        F = m.hann_toneburst(t, 5e6, 3)[None, :]  # one row, broadcast to every entry
+    else:
+        F = np.zeros((p, n_t))
+        for r in range(p):
+            d = int(rng.integers(0, n_t // 3))
+            F[r, d:] = m.hann_toneburst(t[:n_t - d], float(rng.uniform(3e6, 6e6)), 3)
+    q = int(rng.integers(3, 20))
+    hi = rng.choice(n, size=q, replace=True)  # duplicates allowed
+    every = int(rng.choice([0, 17, 40]))
+    mode = MODES[int(rng.integers(0, 2))]
+    return kind, mod, matls, opts, K, C, M, gl, t, F, fi, hi, every, mode
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_case_against_oracle(seed):
+    from bristolfe_v2_amd import ebe, solver
+    kind, mod, matls, opts, K, C, M, gl, t, F, fi, hi, every, mode = _case(seed)
+    print(f"case: random seed {seed} {kind} n={K.shape[0]} n_t={t.size} p={fi.size} "
+          f"rows={F.shape[0]} q={hi.size} every={every} {mode}", flush=True)
+    dt = t[1] - t[0]
+    r, rf, rft, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, F, hi + 1,
+                                                      every if every else np.inf, "KE", mode)
+    assert np.isfinite(r).all() and np.linalg.norm(r) > 0
+    op = ebe.element_operator(mod, matls, opts, gl, K.shape[0])
+    runs = {}
+    with solver.make_plan(K, C, M, dt, mode, operator="csr") as plan:
+        runs["csr"] = plan.run(t.size, fi, F, hi, every, 1)
+        # batched: one column per forcing entry (the entry's own signal row)
+        rows = F if F.shape[0] > 1 else np.repeat(F, fi.size, axis=0)
+        hb, tib = plan.run_batched(t.size, fi.size, fi, np.arange(fi.size), rows, hi)
+        for s in range(fi.size):
+            h1, _, _ = plan.run(t.size, fi[s:s + 1], rows[s:s + 1], hi)
+            assert rel_l2(hb[:, s, :], h1) < 1e-13
+            r1, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi[s:s + 1] + 1, rows[s:s + 1],
+                                                         hi + 1, np.inf, "KE", mode)
+            assert rel_l2(hb[:, s, :], r1) < TOL
+    if op is not None:
+        with solver.make_plan(K, C, M, dt, mode, element_op=op, operator="ebe") as plan:
+            runs["ebe"] = plan.run(t.size, fi, F, hi, every, 1)
+    for name, (h, fld, ti) in runs.items():
+        assert ti == rti, name
+        err = rel_l2(h, r)
+        print(f"  {name}: history rel L2 {err:.2e}", flush=True)
+        assert err < TOL, name
+        if every:
+            assert rel_l2(fld, rf) < TOL, name
