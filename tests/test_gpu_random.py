@@ -3,10 +3,12 @@
 rows), size, absorbing layer, void, solver mode, forcing layout (one row
 broadcast to several entries, or one row per entry with leading zeros and
 duplicate entries, the last duplicate winning), monitors (with duplicates)
-and field output. Each case runs the CSR operator and the element operator
-(FP64, 1e-10 relative L2 on histories and field frames) and the batched
-march (one column per forcing entry) against single-source runs (1e-13)
-and the oracle (1e-10)."""
+and field output; and small C3D4 cubes with random side layers. Each case
+runs the CSR operator and the element operator (FP64, 1e-10 relative L2 on
+histories and field frames; FP32 within 1e-3), and the batched march (one
+column per forcing entry) on the CSR operator, the multi-source element
+product and its node-block form against single-source runs (1e-13) and the
+oracle (1e-10)."""
 import numpy as np
 import pytest
 
@@ -100,6 +102,11 @@ def test_random_case_against_oracle(seed):
     if op is not None:
         with solver.make_plan(K, C, M, dt, mode, element_op=op, operator="ebe") as plan:
             runs["ebe"] = plan.run(t.size, fi, F, hi, every, 1)
+            _batched_operators(plan, t, fi, rows, hi, hb)
+        with solver.make_plan(K, C, M, dt, mode, "single", element_op=op, operator="ebe") as plan:
+            h32, _, _ = plan.run(t.size, fi, F, hi)
+        print(f"  fp32 element operator: rel L2 {rel_l2(h32, r):.2e}", flush=True)
+        assert rel_l2(h32, r) < 1e-3
     for name, (h, fld, ti) in runs.items():
         assert ti == rti, name
         err = rel_l2(h, r)
@@ -107,3 +114,52 @@ def test_random_case_against_oracle(seed):
         assert err < TOL, name
         if every:
             assert rel_l2(fld, rf) < TOL, name
+
+
+def _batched_operators(plan, t, fi, rows, hi, h_csr):
+    """The multi-source element product and the node-block form on the same
+    columns: both within 1e-13 of the CSR batched march (the node-block form
+    where the mesh pattern has it, CSR otherwise)."""
+    from bristolfe_v2_amd import _lib
+    S = fi.size
+    for sel in (_lib.OPERATOR_EBE, _lib.OPERATOR_NODE):
+        plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, sel)
+        h, _ = plan.run_batched(t.size, S, fi, np.arange(S), rows, hi)
+        used = plan.info()["multi_operator"]
+        assert used in (sel, _lib.OPERATOR_CSR)
+        assert rel_l2(h, h_csr) < 1e-13, (sel, used)
+    plan.set_tuning(_lib.TUNE_MULTI_OPERATOR, _lib.OPERATOR_CSR)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_random_tet_cube_against_oracle(seed):
+    """Small C3D4 cubes: random size, element size, side-layer thickness,
+    mode and forcing on the top-face patch, at half the stable step (both
+    modes stable in the layers)."""
+    from bristolfe_v2_amd import configs, ebe, solver
+    rng = np.random.default_rng(100 + seed)
+    # (size, element size) pairs whose cell count is exact in floating point
+    size, el = [(3e-3, 0.25e-3), (2.5e-3, 0.25e-3), (3.5e-3, 0.25e-3), (4e-3, 0.5e-3)][
+        int(rng.integers(0, 4))]
+    layer = float(rng.uniform(0.2, 0.3)) * size
+    mode = MODES[int(rng.integers(0, 2))]
+    n_t = int(rng.integers(100, 180))
+    c0 = configs.cube_3d(size=size, el_size=el, layer=layer, n_t=2, n_mon=4, mon_pitch=el)
+    c = configs.cube_3d(size=size, el_size=el, layer=layer, n_t=n_t, n_mon=4, mon_pitch=el,
+                        dt=0.5 * c0["dt"])
+    K, C, M, t = c["K"], c["C"], c["M"], c["time"]
+    fi = c["frc_gi"] - 1
+    hi = np.unique(np.concatenate([c["hist_gi"] - 1, rng.choice(K.shape[0], 12)]))
+    print(f"case: random tet cube seed {seed} n={K.shape[0]} size={size:.2e} el={el:.2e} "
+          f"{mode}", flush=True)
+    r, _, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, c["frcs"], hi + 1,
+                                                   np.inf, "KE", mode)
+    assert np.isfinite(r).all() and np.linalg.norm(r) > 0
+    op = ebe.element_operator(c["mod"], c["matls"], c["fe_options"], c["gl_lookup"], K.shape[0])
+    assert op is not None
+    for operator in ("csr", "ebe"):
+        with solver.make_plan(K, C, M, c["dt"], mode, element_op=op, operator=operator) as plan:
+            h, _, ti = plan.run(n_t, fi, c["frcs"], hi)
+        err = rel_l2(h, r)
+        print(f"  {operator}: rel L2 {err:.2e}", flush=True)
+        assert ti == rti and err < TOL
