@@ -90,6 +90,15 @@ def test_random_case_against_oracle(seed):
     runs = {}
     with solver.make_plan(K, C, M, dt, mode, operator="csr") as plan:
         runs["csr"] = plan.run(t.size, fi, F, hi, every, 1)
+        # the whole loop as one hipGraph and as one persistent launch: bit-identical
+        from bristolfe_v2_amd import _lib
+        for key in (_lib.TUNE_GRAPH, _lib.TUNE_PERSISTENT):
+            plan.set_tuning(key, 1)
+            hg, fg, tg = plan.run(t.size, fi, F, hi, every, 1)
+            plan.set_tuning(key, 0)
+            np.testing.assert_array_equal(hg, runs["csr"][0])
+            np.testing.assert_array_equal(fg, runs["csr"][1])
+            assert tg == runs["csr"][2]
         # batched: one column per forcing entry (the entry's own signal row)
         rows = F if F.shape[0] > 1 else np.repeat(F, fi.size, axis=0)
         hb, tib = plan.run_batched(t.size, fi.size, fi, np.arange(fi.size), rows, hi)
