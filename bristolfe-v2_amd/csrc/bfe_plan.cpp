@@ -237,12 +237,14 @@ Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& 
         auto it = std::lower_bound(lut.begin(), lut.end(), std::make_pair(g, -1));
         return (it != lut.end() && it->first == g) ? it->second : -1;
       };
+      // the equilibrated block W X W^-1 (W = diag(w)): column l of X^-1 is
+      // w_l W^-1 z with (W X W^-1) z = e_l, and z is what the tests weigh
       std::vector<double> A((size_t)s * s, 0.0);
       for (int a = 0; a < s; ++a) {
         int32_t g = order[a];
         for (int64_t p = X.ptr[g]; p < X.ptr[g + 1]; ++p) {
           int b = loc(X.col[p]);
-          if (b >= 0) A[(size_t)a * s + b] += X.val[p];
+          if (b >= 0) A[(size_t)a * s + b] += X.val[p] * (wt(g) / wt(X.col[p]));
         }
       }
       std::vector<double> x;
@@ -252,7 +254,7 @@ Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& 
       }
       double xmax = 0.0, xbdry = 0.0;
       for (int a = 0; a < s; ++a) {
-        const double xa = std::fabs(x[a]) * wt(order[a]);
+        const double xa = std::fabs(x[a]);
         xmax = std::max(xmax, xa);
         if (depth[a] == radius) xbdry = std::max(xbdry, xa);
       }
@@ -263,7 +265,8 @@ Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& 
       if (exhausted || xbdry <= 1e-20 * xmax) {
         auto& c = cols[l];
         for (int a = 0; a < s; ++a)
-          if (std::fabs(x[a]) * wt(order[a]) > drop_tol * xmax) c.emplace_back(order[a], x[a]);
+          if (std::fabs(x[a]) > drop_tol * xmax)
+            c.emplace_back(order[a], x[a] * (wt(static_cast<int32_t>(l)) / wt(order[a])));
         return;
       }
       radius *= 2;

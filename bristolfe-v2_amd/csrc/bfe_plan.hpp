@@ -91,11 +91,13 @@ Plan build_plan(int64_t n, const CsrRef& K, const double* m, const Csr& C,
 // by column on growing graph neighbourhoods (dense LU with partial pivoting
 // on each neighbourhood) until the neighbourhood boundary carries less than
 // 1e-20 of the column; entries below drop_tol of a column's largest entry
-// are dropped. Both tests weigh entry (a, l) by w[a] (w empty: 1), so that
-// rows of very different scale are judged alike: with X = I + dt/2 M^-1 C
-// and w = sqrt|m| the weighted entries are those of the balanced
-// |M|^1/2 X^-1 |M|^-1/2 (on the fluid-solid interface |m| spans 13 orders
-// of magnitude, and unweighted dropping cost 1e-10 of the histories).
+// are dropped. With weights w (empty: none) the columns are solved on the
+// equilibrated block W X W^-1 (W = diag(w)) and both tests judge its
+// inverse: with X = I + dt/2 M^-1 C and w = sqrt|m| that block is
+// I + dt/2 |M|^-1/2 sign(M) C |M|^-1/2, whose condition number on the
+// fluid-solid interface is about 1 against 1.7e10 for X (|m| spans 13
+// orders of magnitude; without the weights the histories moved by up to
+// 1.5e-10 through dropped entries and by 5e-13 through the solves).
 Csr truncated_inverse(const Csr& X, double drop_tol, const std::vector<double>& w = {});
 
 }  // namespace bfe

@@ -26,8 +26,13 @@ def left_divide(X, Y):
     """X \\ Y for sparse X that is diagonal outside a (small) coupled block.
 
     Block Gaussian elimination: rows/columns with off-diagonal entries of X
-    form the block I, solved densely with LU; every other row is divided by
-    its diagonal. The same linear algebra as MATLAB's sparse '\\'.
+    form the block I, solved densely with LU after scaling each row by the
+    sum of its absolute values; every other row is divided by its diagonal.
+    The same linear algebra as MATLAB's sparse '\\' (UMFPACK, whose default
+    row scaling is that sum). The scaling matters on the fluid-solid
+    interface, where |m| spans 13 orders of magnitude: X_II's condition
+    number is about 1e10 unscaled and 3e5 scaled, and an unscaled LU is off
+    by 1e-12 per solve.
     """
     X = sp.csr_matrix(X)
     Y = sp.csr_matrix(Y)
@@ -46,7 +51,8 @@ def left_divide(X, Y):
     if I.size:
         XII = X[I][:, I].toarray()
         YI = Y[I].toarray()
-        Z = sla.lu_solve(sla.lu_factor(XII), YI)
+        r = 1.0 / np.abs(XII).sum(axis=1)
+        Z = sla.lu_solve(sla.lu_factor(r[:, None] * XII), r[:, None] * YI)
         zr, zc = np.nonzero(Z)
         rows.append(I[zr])
         cols.append(zc)
