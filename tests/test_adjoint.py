@@ -220,3 +220,68 @@ def test_batched_adjoint_node_pairs_match_element_sums(kind, mode):
     else:
         np.testing.assert_array_equal(gk1, gk0)
         np.testing.assert_array_equal(gm1, gm0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", range(4))
+def test_random_batched_adjoint_against_finite_differences(seed):
+    """Seeded random gradient problems: solid or fluid-solid model, solver
+    mode, true-model perturbation, checkpoint interval (partial first
+    segments included), transmitters and receivers. The batched adjoint's
+    stiffness-scale and density gradients against central differences of
+    the oracle's misfit on two sensitive elements (1e-5), and its A-scans
+    against the oracle (1e-10)."""
+    adj = _adj()
+    rng = np.random.default_rng(200 + seed)
+    kind = ["solid", "coupled"][seed % 2]
+    mode = MODES[(seed // 2) % 2]  # every kind in both modes
+    if kind == "solid":
+        mod, matls = _small_solid(absorbing=bool(rng.random() < 0.7))
+        opts = {"dof_to_use": [1, 2], "damping_power_law": 2, "max_damping": 2e7, "solver_mode": mode}
+    else:
+        mod, matls = coupled_block(2e-3, 1.0e-3, 0.75e-3, 0.125e-3)
+        opts = {"dof_to_use": [1, 2, 4], "damping_power_law": 2, "max_damping": 2e7,
+                "solver_mode": mode}
+    pm = adj.ParametrizedModel(mod, matls, opts)
+    dt = 0.8 * PKG.mesh.stable_time_step(pm.K0, pm.M0.diagonal())
+    n_t = int(rng.integers(150, 260))
+    sig = PKG.mesh.hann_toneburst(np.arange(n_t) * dt, float(rng.uniform(4e6, 7e6)), 2)
+    gl, nds = pm.gl, mod["nds"]
+    top = np.nonzero(np.isclose(nds[:, 1], nds[:, 1][gl[:, 1] > 0].max()) & (gl[:, 1] > 0))[0]
+    top = top[np.argsort(nds[top, 0])]
+    n_tx = int(rng.integers(2, 4))
+    tx_nodes = rng.choice(top[1:-1], n_tx, replace=False)
+    tx = [np.array([gl[k, 1] - 1]) for k in tx_nodes]
+    rx = np.array([gl[k, 1] - 1 for k in rng.choice(top, int(rng.integers(3, len(top))), replace=False)])
+    shots = adj.Shots(tx, sig, rx, n_t, dt, ckpt_every=int(rng.integers(7, 61)))
+    print(f"case: random adjoint seed {seed} {kind} {mode} n_t={n_t} tx={n_tx} rx={rx.size} "
+          f"ckpt={shots.ckpt_every}", flush=True)
+    s_true = 1 + 0.05 * rng.standard_normal(pm.n_el)
+    r_true = pm.rho0 * (1 + 0.03 * rng.standard_normal(pm.n_el))
+    K, C, M = pm.matrices(s_true, r_true)
+    t = np.arange(n_t) * dt
+    data = np.stack([ov6.explicit_dynamic_solver_v6(K, C, M, t, x + 1, sig, rx + 1, solver_mode=mode)[0]
+                     for x in tx])
+    s0, r0 = np.ones(pm.n_el), pm.rho0.copy()
+    A, plan, _ = adj.forward_batched(pm, shots, s0, r0, keep=True)
+    gk, gm, _ = adj.backward_batched(plan, A - data)
+    plan.close()
+    K0, C0, M0 = pm.matrices(s0, r0)
+    ref = ov6.explicit_dynamic_solver_v6(K0, C0, M0, t, tx[-1] + 1, sig, rx + 1, solver_mode=mode)[0]
+    assert rel_l2(A[-1], ref) < 1e-10
+    cand = np.argsort(-np.abs(gk))[:30]
+    for e in rng.choice(cand, 2, replace=False):
+        for which, g, base in (("s", gk, s0), ("rho", gm, r0)):
+            h = 1e-4 * abs(base[e])
+            p, m = base.copy(), base.copy()
+            p[e] += h
+            m[e] -= h
+            if which == "s":
+                fd = (_misfit_oracle(pm, shots, data, p, r0, mode) -
+                      _misfit_oracle(pm, shots, data, m, r0, mode)) / (2 * h)
+            else:
+                fd = (_misfit_oracle(pm, shots, data, s0, p, mode) -
+                      _misfit_oracle(pm, shots, data, s0, m, mode)) / (2 * h)
+            err = abs(g[e] - fd) / max(abs(fd), 1e-300)
+            print(f"  {which}[{e}] adjoint {g[e]:.6e} fd {fd:.6e} rel {err:.2e}", flush=True)
+            assert err < 1e-5
