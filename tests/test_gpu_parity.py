@@ -100,6 +100,42 @@ def test_forcing_semantics_on_gpu():
         assert rel_l2(h, r) < TOL
 
 
+def test_empty_and_minimal_marches():
+    """Edge cases of the march against the oracle: no forcing entry (an
+    empty loop: every output 0), no monitor (field frames only), the
+    shortest marches (2 and 3 samples; v6 needs two for dt), a one-row
+    signal non-zero only at its last sample, and a p-row forcing that is all
+    zero (an empty loop again)."""
+    print("case: empty and minimal marches", flush=True)
+    from bristolfe_v2_amd import solver
+    mod, matls = coupled_block(2e-3, 1.0e-3, 0.75e-3, 0.125e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    n = K.shape[0]
+    fi = np.array([gl[3, 1] - 1, gl[7, 0] - 1])
+    hi = np.arange(0, n, 11)
+    with solver.make_plan(K, C, M, dt) as plan:
+        for n_t in (2, 3, 60):
+            t = np.arange(n_t) * dt
+            s = PKG.mesh.hann_toneburst(t, 5e6, 3)
+            cases = [
+                ("no forcing", np.zeros(0, np.int64), np.zeros((1, n_t)), hi),
+                ("no monitors", fi, s[None, :], np.zeros(0, np.int64)),
+                ("last sample only", fi[:1], np.eye(1, n_t, n_t - 1), hi),
+                ("all-zero rows", fi, np.zeros((2, n_t)), hi),
+                ("plain", fi, s[None, :], hi),
+            ]
+            for name, f_i, F, h_i in cases:
+                h, fld, ti = plan.run(n_t, f_i, F, h_i, 1, 1)
+                r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, f_i + 1, F, h_i + 1, 1)
+                assert h.shape == (h_i.size, n_t) and fld.shape == (n, n_t), name
+                assert ti == (rti if rti is not None else n_t + 1), (name, ti, rti)
+                if not np.any(r) and not np.any(rf):
+                    assert not h.any() and not fld.any(), name
+                else:
+                    assert rel_l2(h, r) < TOL and rel_l2(fld, rf) < TOL, name
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_delayed_single_row_field_times(mode):
     """A broadcast toneburst delayed by 40 steps with velocity frames every 7
