@@ -136,6 +136,48 @@ def test_empty_and_minimal_marches():
                     assert rel_l2(h, r) < TOL and rel_l2(fld, rf) < TOL, name
 
 
+def test_batched_edge_cases():
+    """Batched marches with a column that no entry forces (all zero), with
+    no monitor, with 64 columns (the most one launch takes) and delays that
+    differ per column (each column's own first forced step), and the
+    library's refusal of 65 columns."""
+    print("case: batched edge cases", flush=True)
+    from bristolfe_v2_amd import _lib, solver
+    mod, matls = coupled_block(2e-3, 1.0e-3, 0.75e-3, 0.125e-3)
+    K, C, M, gl = assemble(mod, matls, {"dof_to_use": [1, 2, 4], "damping_power_law": 2})
+    dt = PKG.mesh.stable_time_step(K, M.diagonal())
+    n, n_t = K.shape[0], 120
+    t = np.arange(n_t) * dt
+    rng = np.random.default_rng(9)
+    hi = np.arange(0, n, 9)
+    with solver.make_plan(K, C, M, dt) as plan:
+        # three columns, the middle one unforced
+        fd = np.array([5, 40, 77])
+        fs = np.array([0, 2, 2])
+        F = np.vstack([PKG.mesh.hann_toneburst(t - k * dt, 5e6, 3) for k in (0, 10, 30)])
+        hb, _ = plan.run_batched(n_t, 3, fd, fs, F, hi)
+        assert not hb[:, 1, :].any() and hb[:, 0, :].any() and hb[:, 2, :].any()
+        h0, _, _ = plan.run(n_t, fd[:1], F[:1], hi)
+        assert rel_l2(hb[:, 0, :], h0) < 1e-13
+        h2, _, _ = plan.run(n_t, fd[1:], F[1:], hi)
+        assert rel_l2(hb[:, 2, :], h2) < 1e-13
+        # no monitor
+        hq, _ = plan.run_batched(n_t, 3, fd, fs, F, np.zeros(0, np.int64))
+        assert hq.shape == (0, 3, n_t)
+        # 64 columns, one entry each with its own delay
+        S = 64
+        fd64 = rng.choice(n, S, replace=False)
+        F64 = np.vstack([PKG.mesh.hann_toneburst(t - int(rng.integers(0, 40)) * dt, 5e6, 3)
+                         for _ in range(S)])
+        hb64, _ = plan.run_batched(n_t, S, fd64, np.arange(S), F64, hi)
+        for s in (0, 17, 63):
+            r, _, _, _ = ov6.explicit_dynamic_solver_v6(K, C, M, t, fd64[s:s + 1] + 1,
+                                                        F64[s:s + 1], hi + 1)
+            assert rel_l2(hb64[:, s, :], r) < TOL
+        with pytest.raises(_lib.BfeError):
+            plan.run_batched(n_t, 65, fd64[:1], np.zeros(1, np.int64), F64[:1], hi)
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_delayed_single_row_field_times(mode):
     """A broadcast toneburst delayed by 40 steps with velocity frames every 7
