@@ -3,7 +3,8 @@
 rows), size, absorbing layer, void, solver mode, forcing layout (one row
 broadcast to several entries, or one row per entry with leading zeros and
 duplicate entries, the last duplicate winning), monitors (with duplicates)
-and field output; and small C3D4 cubes with random side layers. Each case
+and field output; small C3D4 cubes with random side layers; and jittered
+(unstructured) plates, which the CSR operator marches. Each case
 runs the CSR operator and the element operator (FP64, 1e-10 relative L2 on
 histories and field frames; FP32 within 1e-3), and the batched march (one
 column per forcing entry) on the CSR operator, the multi-source element
@@ -172,3 +173,35 @@ def test_random_tet_cube_against_oracle(seed):
         err = rel_l2(h, r)
         print(f"  {operator}: rel L2 {err:.2e}", flush=True)
         assert ti == rti and err < TOL
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_random_unstructured_mesh(seed):
+    """A plate whose nodes are jittered (no two elements alike): the element
+    operator does not apply, make_plan("auto") marches the CSR operator, and
+    the histories match the oracle."""
+    from bristolfe_v2_amd import ebe, solver
+    rng = np.random.default_rng(300 + seed)
+    m = PKG.mesh
+    w, h, el = 3e-3, 1.5e-3, 0.1e-3
+    mod = m.fn_rectangular_structured_mesh(rect(w, h), el)
+    mod["nds"] = mod["nds"] + rng.uniform(-0.25, 0.25, mod["nds"].shape) * el
+    mod["el_typ_i"] = np.array(["CPE3"] * len(mod["el_mat_i"]), dtype=object)
+    inner = np.array([[0.6e-3, 0.4e-3], [2.4e-3, 0.4e-3], [2.4e-3, 5e-3], [0.6e-3, 5e-3]])
+    mod = m.fn_add_absorbing_layer(mod, inner, 0.4e-3)
+    opts = {"dof_to_use": [1, 2], "damping_power_law": 2}
+    K, C, M, gl = assemble(mod, [steel()], opts)
+    assert ebe.element_operator(mod, [steel()], opts, gl, K.shape[0]) is None
+    dt = 0.5 * m.stable_time_step(K, M.diagonal())
+    t = np.arange(250) * dt
+    mode = MODES[seed % 2]
+    fi = rng.choice(K.shape[0], 2, replace=False)
+    F = m.hann_toneburst(t, 5e6, 3)[None, :]
+    hi = rng.choice(K.shape[0], 15, replace=False)
+    print(f"case: random unstructured seed {seed} n={K.shape[0]} {mode}", flush=True)
+    with solver.make_plan(K, C, M, dt, mode, operator="auto") as plan:
+        assert plan.info()["operator_kind"] == 0
+        h, fld, ti = plan.run(t.size, fi, F, hi, 50, 1)
+    r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, F, hi + 1, 50, "KE", mode)
+    assert ti == rti and np.linalg.norm(r) > 0
+    assert rel_l2(h, r) < TOL and rel_l2(fld, rf) < TOL
