@@ -3,8 +3,9 @@
 rows), size, absorbing layer, void, solver mode, forcing layout (one row
 broadcast to several entries, or one row per entry with leading zeros and
 duplicate entries, the last duplicate winning), monitors (with duplicates)
-and field output; small C3D4 cubes with random side layers; and jittered
-(unstructured) plates, which the CSR operator marches. Each case
+and field output; small C3D4 cubes with random side layers; jittered
+(unstructured) plates, which the CSR operator marches; and the device
+assembly of jittered meshes against the numpy assembler. Each case
 runs the CSR operator and the element operator (FP64, 1e-10 relative L2 on
 histories and field frames; FP32 within 1e-3), and the batched march (one
 column per forcing entry) on the CSR operator, the multi-source element
@@ -205,3 +206,40 @@ def test_random_unstructured_mesh(seed):
     r, rf, _, rti = ov6.explicit_dynamic_solver_v6(K, C, M, t, fi + 1, F, hi + 1, 50, "KE", mode)
     assert ti == rti and np.linalg.norm(r) > 0
     assert rel_l2(h, r) < TOL and rel_l2(fld, rf) < TOL
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_device_assembly(seed):
+    """The device assembly (bfe_assemble_device) against the numpy assembler
+    on jittered (unstructured) versions of the random meshes above and of
+    small C3D4 cubes with random absorption: identical DOF numbering, values
+    to 1e-13."""
+    rng = np.random.default_rng(400 + seed)
+    if seed < 6:
+        kind, mod, matls, opts, *_ = _case(seed)
+        mod = dict(mod)
+        el = float(np.min(np.abs(np.diff(np.unique(np.round(mod["nds"][:, 0], 12))))))
+    else:
+        kind, el = "tets", 0.25e-3
+        mod = PKG.mesh.cubic_structured_tet_mesh(float(rng.choice([0.75e-3, 1e-3])), el)
+        mod["el_typ_i"] = np.array(["C3D4"] * len(mod["el_mat_i"]), dtype=object)
+        mod["el_abs_i"] = rng.uniform(0, 1, len(mod["el_mat_i"])) * (rng.random(len(mod["el_mat_i"])) < 0.5)
+        matls = [steel()]
+        opts = {"dof_to_use": [] if seed == 6 else [1, 2, 3], "damping_power_law": 2}
+    # each coordinate moved by up to a fifth of the smallest spacing: no
+    # element turns over (checked on the host below)
+    mod["nds"] = mod["nds"] + rng.uniform(-0.2, 0.2, mod["nds"].shape) * el
+    if kind == "tets":
+        e = np.asarray(mod["els"], np.int64)[:, :4] - 1
+        x = mod["nds"]
+        vol = np.einsum("ij,ij->i", np.cross(x[e[:, 1]] - x[e[:, 0]], x[e[:, 2]] - x[e[:, 0]]),
+                        x[e[:, 3]] - x[e[:, 0]])
+        assert np.all(vol != 0) and len(np.unique(np.sign(vol))) == 1
+    print(f"case: random device assembly seed {seed} {kind} n_el={len(mod['el_mat_i'])}", flush=True)
+    K1, C1, M1, gl1 = assemble(mod, matls, dict(opts, assembly_backend="numpy"))
+    K2, C2, M2, gl2 = assemble(mod, matls, dict(opts, assembly_backend="device"))
+    np.testing.assert_array_equal(gl1, gl2)
+    for A, B in ((K1, K2), (C1, C2), (M1, M2)):
+        assert A.shape == B.shape
+        scale = max(abs(A).max(), abs(B).max(), 1e-300)
+        assert abs(A - B).max() <= 1e-13 * scale
