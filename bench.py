@@ -80,7 +80,11 @@ def copy_probe(gib=2.0):
     """Kernels of known traffic, for calibrating rocprofv3's FETCH_SIZE /
     WRITE_SIZE counters (run under rocprofv3 --pmc): a plain device copy
     (reads and writes `gib` GiB), a sum (reads `gib` GiB, writes 8 bytes per
-    workgroup) and a fill (writes `gib` GiB), three dispatches each."""
+    workgroup), a fill (writes `gib` GiB), and two row gathers in a seeded
+    random order (index_select, every source row read once): 512-byte rows
+    (64 FP64, a 64-column multivector's row) and 64-byte rows (8 FP64), each
+    moving gib/2 GiB in and out plus the 8-byte indices; three dispatches
+    each."""
     import torch
     n = int(gib * (1 << 30)) // 8
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -94,7 +98,18 @@ def copy_probe(gib=2.0):
     for _ in range(3):
         b.fill_(2.0)
     torch.cuda.synchronize()
-    print(json.dumps({"probe_bytes": n * 8, "dispatches_each": 3, "sum": s}))
+    g = torch.Generator(device="cpu").manual_seed(7)
+    rows = {}
+    for width in (64, 8):
+        nr = n // 2 // width
+        src = a[:nr * width].view(nr, width)
+        idx = torch.randperm(nr, generator=g).to("cuda")
+        out = b[:nr * width].view(nr, width)
+        for _ in range(3):
+            torch.index_select(src, 0, idx, out=out)
+        torch.cuda.synchronize()
+        rows[f"gather_{width * 8}B_rows"] = nr
+    print(json.dumps({"probe_bytes": n * 8, "dispatches_each": 3, "sum": s, **rows}))
 
 
 def algorithmic_bytes_per_dof(nnz, n, damped):
@@ -690,7 +705,7 @@ def main():
     ap.add_argument("--time-steps", type=int, default=None,
                     help="override the workload's number of time steps (profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only run a 2 GiB device copy (counter calibration)")
+                    help="only run the counter-calibration kernels (2 GiB copy, sum, fill, row gathers)")
     ap.add_argument("--fmc-shape", default=None,
                     help="fmc_batched: fixed launch shape rows,rpw,remap,group,tile[,npw] instead "
                          "of the autotuner (npw > 0: the multi-source element product with npw "
